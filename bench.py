@@ -1,0 +1,250 @@
+"""Benchmark: 30-qubit complex128 random circuit, fused k-qubit HBM passes.
+
+Workload (one "step"): the 30-qubit, 40-layer random circuit (each layer: one
+of {H, Rx, Ry, Rz} per qubit with random angles, then CNOT ladders), 1780
+gates, fused into passes of at most k qubits (default k=4), with 8 uniformly
+random qubit-index permutations (the in-HBM replacement of the reference's
+qswap) at fixed layer boundaries.  The state (16 GiB) stays resident in HBM;
+scheduling/fusion-cluster planning happens before the timed region.
+
+metric value = algorithmic HBM bytes of all passes in the step (32 B per
+amplitude per fused or permutation pass) / step wall time.
+roofline     = the dominant kernel (dense k-qubit pass): 32 * 2^30 B per
+               launch / its average launch duration from HIP events recorded
+               on the engine's stream over the timed region.
+cpu_baseline = the reference's own AVX2 kernels (oracle/_ref, OpenMP) on the
+               host, k-qubit passes over a 2^28-amplitude complex128 state.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--k 4] [--qubits 30]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def plan_passes(gates, k):
+    """In program order: a new pass starts when the next gate would widen the
+    current one beyond k qubits (targets + controls)."""
+    passes, cur, cur_q = [], [], set()
+    for g in gates:
+        q = set(g[1]) | set(g[2])
+        if cur and len(cur_q | q) > k:
+            passes.append(cur)
+            cur, cur_q = [], set()
+        cur.append(g)
+        cur_q |= q
+    if cur:
+        passes.append(cur)
+    return passes
+
+
+def build_step(n, layers, k, seed, n_perm):
+    from hiqsimulator_amd.circuits import random_layers_circuit, random_permutation
+    rng = np.random.default_rng(seed + 1)
+    gates = random_layers_circuit(n, layers, seed)
+    # Permutations after every (layers // n_perm)-th layer boundary.
+    bounds = set()
+    if n_perm:
+        stride = max(1, layers // n_perm)
+        bounds = {stride * (i + 1) for i in range(n_perm)}
+    ops, idx = [], 0
+    ids = list(range(n))
+    # Walk layers (gate counts per layer alternate: n + n//2, n + (n-1)//2).
+    for layer in range(layers):
+        cnt = n + len(range(layer % 2, n - 1, 2))
+        chunk = gates[idx:idx + cnt]
+        idx += cnt
+        ops.append(("gates", chunk))
+        if (layer + 1) in bounds:
+            perm = random_permutation(ids, rng)
+            ops.append(("perm", perm))
+    assert idx == len(gates), (idx, len(gates))
+    # Fuse within each stretch between permutations.
+    step = []
+    stretch = []
+    for kind, payload in ops:
+        if kind == "gates":
+            stretch.extend(payload)
+        else:
+            step.extend(("pass", p) for p in plan_passes(stretch, k))
+            stretch = []
+            step.append(("perm", payload))
+    step.extend(("pass", p) for p in plan_passes(stretch, k))
+    return gates, step
+
+
+def run_step(sim, step):
+    for kind, payload in step:
+        if kind == "pass":
+            for m, t, c in payload:
+                sim.apply_controlled_gate(m, t, c)
+            sim.run()
+        else:
+            sim.permute_qubits(payload)
+
+
+def kernel_sweep(hq, n, reps=6):
+    """Per-pass GB/s of the dense kernel for k = 1..5 at random positions."""
+    from hiqsimulator_amd.circuits import haar_unitary
+    rng = np.random.default_rng(5)
+    sim = hq.SimulatorMPI(1, n, 5)
+    sim.allocate_qureg(list(range(n)), 0)
+    out = {}
+    for k in range(1, 6):
+        mats = [haar_unitary(1 << k, rng) for _ in range(reps)]
+        poss = [[int(p) for p in rng.permutation(n)[:k]] for _ in range(reps)]
+        sim.apply_matrix(mats[0], poss[0])  # warm
+        sim.synchronize()
+        sim.set_timing(True)
+        for m, p in zip(mats, poss):
+            sim.apply_matrix(m, p)
+        cnt, ms = sim.kernel_time(k - 1)
+        sim.set_timing(False)
+        out[str(k)] = round(32.0 * 2 ** n / (ms / cnt * 1e-3) / 1e9, 1)
+    del sim
+    return out
+
+
+def cpu_baseline(k, seconds=12.0, n_sample=28):
+    try:
+        import oracle
+        from hiqsimulator_amd.circuits import haar_unitary
+    except Exception as exc:  # pragma: no cover
+        return {"value": None, "unit": "GB/s", "error": str(exc)}
+    rng = np.random.default_rng(3)
+    if oracle.RefKernels.available("intrin"):
+        ref = oracle.RefKernels("intrin")
+        kind, cores = "reference", ref.num_threads()
+        h, arr = ref.new_state(1 << n_sample)
+        arr.fill(2.0 ** (-n_sample / 2))
+        apply = lambda m, pos: ref.apply(h, m, pos)  # noqa: E731
+    else:
+        kind, cores, n_sample = "port", 1, 22
+        arr = np.full(1 << n_sample, 2.0 ** (-n_sample / 2), dtype=np.complex128)
+        h = None
+        apply = lambda m, pos: oracle.dense_pass(arr, m, pos)  # noqa: E731
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        m = haar_unitary(1 << k, rng)
+        pos = [int(p) for p in rng.permutation(n_sample)[:k]]
+        apply(m, pos)
+        passes += 1
+        el = time.perf_counter() - t0
+        if (el > seconds and passes >= 3) or passes >= 400:
+            break
+    if h is not None:
+        ref.free_state(h)
+    gbs = 32.0 * 2 ** n_sample * passes / el / 1e9
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores, "kind": kind,
+            "sample": "%d dense k=%d passes over a 2^%d-amplitude complex128 state, %.1f s"
+                      % (passes, k, n_sample, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--k", type=int, default=4)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--perms", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import hiqsimulator_amd as hq
+    n, k = args.qubits, args.k
+    t0 = time.perf_counter()
+    gates, step = build_step(n, args.layers, k, args.seed, args.perms)
+    n_pass = sum(1 for s in step if s[0] == "pass")
+    n_perm = sum(1 for s in step if s[0] == "perm")
+    plan_s = time.perf_counter() - t0
+    log("plan: %d gates -> %d fused passes + %d permutations (%.2f s)"
+        % (len(gates), n_pass, n_perm, plan_s))
+
+    sim = hq.SimulatorMPI(args.seed, n, k)
+    sim.allocate_qureg(list(range(n)), 0)
+    sim.synchronize()
+    for w in range(args.warmup):
+        run_step(sim, step)
+        sim.synchronize()
+        log("warmup %d done" % w)
+    sim.set_timing(True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        run_step(sim, step)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    log("timed %d steps in %.3f s" % (args.steps, el))
+    times = {name: sim.kernel_time(i) for i, name in enumerate(hq.KERNEL_CLASSES)}
+    sim.set_timing(False)
+    norm = sim.get_probability([], [])
+    bytes_step = 32.0 * 2 ** n * (n_pass + n_perm)
+    ms_step = el / args.steps * 1e3
+    value = bytes_step * args.steps / el / 1e9
+
+    dom = "dense_k%d" % k
+    cnt, tot = times[dom]
+    per_launch_ms = tot / cnt if cnt else float("nan")
+    achieved = 32.0 * 2 ** n / (per_launch_ms * 1e-3) / 1e9
+    kernel_share = {name: round(ms, 2) for name, (c, ms) in times.items() if c}
+    del sim
+
+    sweep = None if args.no_sweep else kernel_sweep(hq, n)
+    if sweep:
+        log("per-pass GB/s by k: %s" % sweep)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(k)
+    if cpu:
+        log("cpu baseline: %s" % cpu)
+
+    line = {
+        "metric": "30q fp64 fused-pass HBM GB/s vs peak; mixed-size circuits/s + p99 latency",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random circuit)",
+        "config": {"workload": "%dq complex128 random circuit, %d layers, %d gates, fused k=%d "
+                               "(%d passes) + %d qubit permutations" %
+                               (n, args.layers, len(gates), k, n_pass, n_perm),
+                   "qubits": n, "layers": args.layers, "fusion_k": k, "passes_per_step": n_pass,
+                   "permutations_per_step": n_perm, "plan_seconds": round(plan_s, 3),
+                   "final_norm": norm},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": dom, "launches": cnt, "avg_launch_ms": round(per_launch_ms, 4)},
+        "kernel_ms_total": kernel_share,
+        "k_sweep_gbs": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,778 @@
+// Host side of the single-GPU state-vector engine.  See engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace hq {
+namespace {
+
+constexpr double kMaxFloatError = 1e-12;  // SimulatorMPI.cpp:75
+constexpr uint64_t kMaxBlocks = uint64_t(1) << 15;  // SimulatorMPI.cpp:903
+constexpr int kMaxOuts = 256;                        // batched reduction outputs
+
+void check(hipError_t e, const char* what)
+{
+     if (e != hipSuccess) {
+          std::ostringstream os;
+          os << what << ": " << hipGetErrorString(e);
+          throw std::runtime_error(os.str());
+     }
+}
+
+std::string id_list(const std::vector<int64_t>& v)
+{
+     std::ostringstream os;
+     os << "[";
+     for (size_t i = 0; i < v.size(); ++i) os << (i ? ", " : "") << v[i];
+     os << "]";
+     return os.str();
+}
+
+struct DeviceScope {
+     explicit DeviceScope(int dev) { check(hipSetDevice(dev), "hipSetDevice"); }
+};
+
+}  // namespace
+
+Engine::Engine(uint64_t seed, int64_t max_local, int64_t max_cluster, Prec prec, int device)
+    : prec_(prec),
+      device_(device),
+      max_local_(static_cast<size_t>(std::max<int64_t>(0, max_local))),
+      max_cluster_(static_cast<size_t>(std::max<int64_t>(0, max_cluster))),
+      min_local_(static_cast<size_t>(std::max<int64_t>(0, max_cluster))),
+      // std::mt19937 keeps the seed modulo 2^32 (SimulatorMPI.cpp:88).
+      rng_(static_cast<std::mt19937::result_type>(seed))
+{
+     if (max_local < 0 || max_cluster < 0)
+          throw std::runtime_error("SimulatorMPI(): max_local and max_cluster_size must be >= 0");
+     int ndev = 0;
+     check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+     if (device < 0 || device >= ndev)
+          throw std::runtime_error("SimulatorMPI(): no HIP device " + std::to_string(device));
+     DeviceScope ds(device_);
+     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+     check(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), kRing * kSlotBytes,
+                         hipHostMallocDefault),
+           "hipHostMalloc(ring)");
+     check(hipMalloc(reinterpret_cast<void**>(&d_ring_), kRing * kSlotBytes), "hipMalloc(ring)");
+     for (int i = 0; i < kRing; ++i)
+          check(hipEventCreateWithFlags(&ring_ev_[i], hipEventDisableTiming), "hipEventCreate");
+     check(hipMalloc(reinterpret_cast<void**>(&d_scratch_),
+                     (kReduceScratchDoubles + 2 * kMaxOuts) * sizeof(double)),
+           "hipMalloc(scratch)");
+     check(hipHostMalloc(reinterpret_cast<void**>(&h_out_), 2 * kMaxOuts * sizeof(double),
+                         hipHostMallocDefault),
+           "hipHostMalloc(out)");
+     check(hipMalloc(reinterpret_cast<void**>(&d_blocks_), kMaxBlocks * sizeof(double)),
+           "hipMalloc(blocks)");
+     check(hipHostMalloc(reinterpret_cast<void**>(&h_blocks_), kMaxBlocks * sizeof(double),
+                         hipHostMallocDefault),
+           "hipHostMalloc(blocks)");
+     // The empty register is the one-amplitude state [1] (SimulatorMPI.cpp:96-98).
+     ensure_state(1);
+     check(launch_fill(d_state_, prec_, 1, 1.0, 0.0, stream_), "fill");
+     n_amps_ = 1;
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+Engine::~Engine()
+{
+     (void)hipSetDevice(device_);
+     if (stream_) (void)hipStreamSynchronize(stream_);
+     for (auto& p: pending_) {
+          (void)hipEventDestroy(p.start);
+          (void)hipEventDestroy(p.stop);
+     }
+     for (auto e: free_events_) (void)hipEventDestroy(e);
+     for (int i = 0; i < kRing; ++i)
+          if (ring_ev_[i]) (void)hipEventDestroy(ring_ev_[i]);
+     if (d_state_) (void)hipFree(d_state_);
+     if (d_tmp_) (void)hipFree(d_tmp_);
+     if (d_ring_) (void)hipFree(d_ring_);
+     if (h_ring_) (void)hipHostFree(h_ring_);
+     if (d_scratch_) (void)hipFree(d_scratch_);
+     if (h_out_) (void)hipHostFree(h_out_);
+     if (d_blocks_) (void)hipFree(d_blocks_);
+     if (h_blocks_) (void)hipHostFree(h_blocks_);
+     if (h_stage_) (void)hipHostFree(h_stage_);
+     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ---------------------------------------------------------------------------
+// Buffers
+// ---------------------------------------------------------------------------
+void Engine::ensure_state(uint64_t n)
+{
+     if (n <= cap_state_) return;
+     void* p = nullptr;
+     check(hipMalloc(&p, n * amp_bytes(prec_)), "hipMalloc(state)");
+     if (d_state_) {
+          check(launch_copy(p, d_state_, prec_, n_amps_, stream_), "copy");
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          check(hipFree(d_state_), "hipFree");
+     }
+     d_state_ = p;
+     cap_state_ = n;
+}
+
+void Engine::ensure_tmp(uint64_t n)
+{
+     if (n <= cap_tmp_) return;
+     if (d_tmp_) {
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          check(hipFree(d_tmp_), "hipFree");
+          d_tmp_ = nullptr;
+          cap_tmp_ = 0;
+     }
+     check(hipMalloc(&d_tmp_, n * amp_bytes(prec_)), "hipMalloc(tmp)");
+     cap_tmp_ = n;
+}
+
+void Engine::swap_tmp()
+{
+     std::swap(d_state_, d_tmp_);
+     std::swap(cap_state_, cap_tmp_);
+}
+
+const void* Engine::upload_matrix(const Mat& m)
+{
+     const int slot = ring_next_;
+     ring_next_ = (ring_next_ + 1) % kRing;
+     if (ring_used_[slot]) check(hipEventSynchronize(ring_ev_[slot]), "hipEventSynchronize");
+     char* h = h_ring_ + slot * kSlotBytes;
+     const size_t n = m.dim * m.dim;
+     size_t bytes;
+     if (prec_ == Prec::C128) {
+          auto* o = reinterpret_cast<double*>(h);
+          for (size_t i = 0; i < n; ++i) {
+               o[2 * i] = m.a[i].real();
+               o[2 * i + 1] = m.a[i].imag();
+          }
+          bytes = n * 16;
+     }
+     else {
+          auto* o = reinterpret_cast<float*>(h);
+          for (size_t i = 0; i < n; ++i) {
+               o[2 * i] = static_cast<float>(m.a[i].real());
+               o[2 * i + 1] = static_cast<float>(m.a[i].imag());
+          }
+          bytes = n * 8;
+     }
+     char* d = d_ring_ + slot * kSlotBytes;
+     check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(matrix)");
+     check(hipEventRecord(ring_ev_[slot], stream_), "hipEventRecord");
+     ring_used_[slot] = true;
+     return d;
+}
+
+void Engine::read_out(double* host, size_t count)
+{
+     check(hipMemcpyAsync(host, d_scratch_ + kReduceScratchDoubles, count * sizeof(double),
+                          hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync(out)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------------------
+// Timing
+// ---------------------------------------------------------------------------
+void Engine::begin_timed(int cls)
+{
+     if (!timing_) return;
+     hipEvent_t ev[2];
+     for (auto& e: ev) {
+          if (!free_events_.empty()) {
+               e = free_events_.back();
+               free_events_.pop_back();
+          }
+          else {
+               check(hipEventCreate(&e), "hipEventCreate");
+          }
+     }
+     check(hipEventRecord(ev[0], stream_), "hipEventRecord");
+     pending_.push_back(TimedLaunch{cls, ev[0], ev[1]});
+}
+
+void Engine::end_timed()
+{
+     if (!timing_) return;
+     check(hipEventRecord(pending_.back().stop, stream_), "hipEventRecord");
+}
+
+void Engine::set_timing(bool on)
+{
+     DeviceScope ds(device_);
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     timing_ = on;
+     for (int c = 0; c < kNumKernelClasses; ++c) {
+          t_count_[c] = 0;
+          t_ms_[c] = 0;
+     }
+     for (auto& p: pending_) {
+          free_events_.push_back(p.start);
+          free_events_.push_back(p.stop);
+     }
+     pending_.clear();
+}
+
+void Engine::kernel_time(int cls, uint64_t& count, double& total_ms)
+{
+     DeviceScope ds(device_);
+     if (cls < 0 || cls >= kNumKernelClasses) throw std::runtime_error("kernel_time(): bad class");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     for (auto& p: pending_) {
+          float ms = 0;
+          check(hipEventElapsedTime(&ms, p.start, p.stop), "hipEventElapsedTime");
+          t_count_[p.cls] += 1;
+          t_ms_[p.cls] += ms;
+          free_events_.push_back(p.start);
+          free_events_.push_back(p.stop);
+     }
+     pending_.clear();
+     count = t_count_[cls];
+     total_ms = t_ms_[cls];
+}
+
+void Engine::synchronize()
+{
+     DeviceScope ds(device_);
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------------------
+// Qubit bookkeeping
+// ---------------------------------------------------------------------------
+size_t Engine::find(int64_t id) const
+{
+     auto it = std::find(locals_.begin(), locals_.end(), id);
+     return it == locals_.end() ? static_cast<size_t>(-1) : static_cast<size_t>(it - locals_.begin());
+}
+
+size_t Engine::find_sure(int64_t id) const
+{
+     size_t p = find(id);
+     if (p == static_cast<size_t>(-1))
+          throw std::runtime_error("ArrayFindSure(): Can't find " + std::to_string(id) + " in "
+                                   + id_list(locals_));
+     return p;
+}
+
+void Engine::allocate_qubit(int64_t id) { allocate_qureg({id}, cplx(0.0, 0.0)); }
+
+void Engine::allocate_qureg(const std::vector<int64_t>& ids, cplx init)
+{
+     DeviceScope ds(device_);
+     if (init != 0.0 && !locals_.empty())
+          throw std::runtime_error(
+              "AllocateQureg(): initialization of only first qureg is supported");
+     // A qubit can be allocated while fewer than max(min_local, max_local)
+     // are present (SimulatorMPI.cpp:190-212 with no global slots).
+     const size_t limit = std::max(min_local_, max_local_);
+     const size_t room = locals_.size() < limit ? limit - locals_.size() : 0;
+     const size_t count = std::min(room, ids.size());
+     if (count > 0) {
+          if (locals_.size() + count > 62)
+               throw std::runtime_error("AllocateQubit(): state vector too large");
+          const uint64_t n_new = n_amps_ << count;
+          ensure_state(n_new);
+          begin_timed(kFillCopy);
+          check(launch_fill(static_cast<char*>(d_state_) + n_amps_ * amp_bytes(prec_), prec_,
+                            n_new - n_amps_, 0.0, 0.0, stream_),
+                "fill");
+          end_timed();
+          n_amps_ = n_new;
+          locals_.insert(locals_.end(), ids.begin(), ids.begin() + static_cast<long>(count));
+     }
+     if (count < ids.size())
+          throw std::runtime_error("AllocateQubit(): can't allocate more than "
+                                   + std::to_string(locals_.size()) + " qubits");
+     if (init != 0.0) {
+          begin_timed(kFillCopy);
+          check(launch_fill(d_state_, prec_, n_amps_, init.real(), init.imag(), stream_), "fill");
+          end_timed();
+     }
+}
+
+void Engine::deallocate_qubit(int64_t id)
+{
+     DeviceScope ds(device_);
+     const size_t pos = find_sure(id);
+     begin_timed(kReduce);
+     check(launch_two_bucket(d_state_, prec_, n_qubits(), static_cast<int>(pos), d_scratch_,
+                             d_scratch_ + kReduceScratchDoubles, stream_),
+           "two_bucket");
+     end_timed();
+     read_out(h_out_, 2);
+     const double s0 = h_out_[0], s1 = h_out_[1];
+     if (!((s0 > kMaxFloatError) ^ (s1 > kMaxFloatError)))
+          throw std::runtime_error("DeallocateLocalQubit(): qubit " + std::to_string(id)
+                                   + " is entangled");
+     const int keep = s0 > kMaxFloatError ? 0 : 1;
+     ensure_tmp(n_amps_ / 2);
+     begin_timed(kRemoveBit);
+     check(launch_remove_bit(d_tmp_, d_state_, prec_, n_amps_, static_cast<int>(pos), keep,
+                             stream_),
+           "remove_bit");
+     end_timed();
+     swap_tmp();
+     n_amps_ /= 2;
+     locals_.erase(locals_.begin() + static_cast<long>(pos));
+}
+
+void Engine::set_qubits_perm(const std::vector<int64_t>& ids)
+{
+     if (ids.size() != locals_.size())
+          throw std::runtime_error("SetQubitsPermutation(): expected "
+                                   + std::to_string(locals_.size()) + " ids");
+     locals_ = ids;
+}
+
+void Engine::swap_qubits(const std::vector<int64_t>& pairs)
+{
+     // With one device every qubit is local: any (global, local) pair names a
+     // qubit that is not global (SimulatorMPI.cpp:1092-1096).
+     if (pairs.empty()) return;
+     throw std::runtime_error("ArrayFindSure(): Can't find " + std::to_string(pairs[0])
+                              + " in [] (no global qubits on a single device)");
+}
+
+void Engine::permute_qubits(const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     if (ids.size() != locals_.size())
+          throw std::runtime_error("permute_qubits(): expected " + std::to_string(locals_.size())
+                                   + " ids");
+     std::vector<uint8_t> src(ids.size());
+     uint64_t seen = 0;
+     bool identity = true;
+     for (size_t p = 0; p < ids.size(); ++p) {
+          const size_t s = find_sure(ids[p]);
+          src[p] = static_cast<uint8_t>(s);
+          seen |= uint64_t(1) << s;
+          identity = identity && s == p;
+     }
+     if (seen != (ids.empty() ? 0 : ((uint64_t(1) << ids.size()) - 1)))
+          throw std::runtime_error("permute_qubits(): ids must be a permutation of the qubits");
+     if (identity) return;
+     ensure_tmp(n_amps_);
+     begin_timed(kPermute);
+     check(launch_permute(d_tmp_, d_state_, prec_, n_qubits(), src.data(), stream_), "permute");
+     end_timed();
+     swap_tmp();
+     locals_ = ids;
+}
+
+// ---------------------------------------------------------------------------
+// Gates
+// ---------------------------------------------------------------------------
+void Engine::apply_gate(Mat m, std::vector<int64_t> ids, const std::vector<int64_t>& ctrls)
+{
+     if (ids.size() > 62 || m.dim != (size_t(1) << ids.size()))
+          throw std::runtime_error("apply_controlled_gate(): a " + std::to_string(m.dim) + "x"
+                                   + std::to_string(m.dim) + " matrix cannot act on "
+                                   + std::to_string(ids.size()) + " qubits");
+     const bool diag = is_diagonal(m);
+     std::vector<int64_t> lc;
+     for (auto c: ctrls)
+          if (find(c) != static_cast<size_t>(-1)) lc.push_back(c);
+     if (ids.size() + lc.size() > max_cluster_) {
+          // A gate wider than a cluster runs alone with a control mask
+          // (SimulatorMPI.cpp:730-756).
+          run();
+          fusion_.insert(std::move(m), diag, std::move(ids), ctrls);
+          return;
+     }
+     add_controls(m, ids, lc);
+     fusion_.insert(std::move(m), diag, std::move(ids), {});
+}
+
+void Engine::run()
+{
+     DeviceScope ds(device_);
+     Mat m;
+     std::vector<int64_t> ids, ctrls;
+     bool diag = false;
+     fusion_.perform(m, ids, ctrls, diag);
+     uint64_t cmask = 0;
+     for (auto c: ctrls) {
+          const size_t p = find(c);
+          if (p != static_cast<size_t>(-1)) cmask |= uint64_t(1) << p;
+     }
+     std::vector<uint32_t> pos(ids.size());
+     for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
+     if (ids.empty()) {
+          if (m(0, 0) != cplx(1.0, 0.0)) {
+               begin_timed(kScale);
+               check(launch_scale(d_state_, prec_, n_amps_, m(0, 0).real(), m(0, 0).imag(),
+                                  stream_),
+                     "scale");
+               end_timed();
+          }
+     }
+     else if (ids.size() <= 5) {
+          dense_or_diag(m, pos, cmask, diag);
+     }
+     else {
+          throw std::runtime_error("Run(): cannot apply " + std::to_string(ids.size())
+                                   + " qubits gate");
+     }
+     fusion_ = Fusion();
+}
+
+void Engine::dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
+                           bool diag)
+{
+     DenseDesc d{};
+     d.k = static_cast<int>(pos.size());
+     uint64_t all = ctrlmask;
+     for (size_t l = 0; l < pos.size(); ++l) {
+          d.pos[l] = pos[l];
+          all |= uint64_t(1) << pos[l];
+     }
+     d.ctrlmask = ctrlmask;
+     for (int p = 0; p < 64; ++p)
+          if ((all >> p) & 1) {
+               if (d.ins.n >= 63) throw std::runtime_error("Run(): too many controls");
+               d.ins.p[d.ins.n++] = static_cast<uint8_t>(p);
+          }
+     if (diag) {
+          double dv[64];
+          for (size_t i = 0; i < m.dim; ++i) {
+               dv[2 * i] = m(i, i).real();
+               dv[2 * i + 1] = m(i, i).imag();
+          }
+          begin_timed(kDiag);
+          check(launch_diag(d_state_, prec_, n_qubits(), dv, d, stream_), "diag pass");
+          end_timed();
+     }
+     else {
+          const void* dm = upload_matrix(m);
+          begin_timed(kDense1 + d.k - 1);
+          check(launch_dense(d_state_, prec_, n_qubits(), dm, d, stream_), "dense pass");
+          end_timed();
+     }
+}
+
+void Engine::apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
+                          bool diag)
+{
+     DeviceScope ds(device_);
+     const size_t k = pos.size();
+     if (k < 1 || k > 5 || m.dim != (size_t(1) << k))
+          throw std::runtime_error("apply_matrix(): need 1..5 positions and a matching matrix");
+     uint64_t tm = 0;
+     for (auto p: pos) {
+          if (p >= locals_.size() || ((tm >> p) & 1))
+               throw std::runtime_error("apply_matrix(): bad or repeated bit position");
+          tm |= uint64_t(1) << p;
+     }
+     if ((ctrlmask & tm) || (locals_.size() < 64 && (ctrlmask >> locals_.size())))
+          throw std::runtime_error("apply_matrix(): bad control mask");
+     dense_or_diag(m, pos, ctrlmask, diag && is_diagonal(m));
+}
+
+// ---------------------------------------------------------------------------
+// Measurement, probabilities, amplitudes
+// ---------------------------------------------------------------------------
+void Engine::mask_of(const std::vector<int64_t>& ids, const std::vector<uint8_t>& bits,
+                     uint64_t& msk, uint64_t& val) const
+{
+     msk = val = 0;
+     for (size_t i = 0; i < ids.size(); ++i) {
+          const size_t p = find_sure(ids[i]);
+          msk |= uint64_t(1) << p;
+          if (bits[i]) val |= uint64_t(1) << p;
+     }
+}
+
+double Engine::masked_norm(uint64_t msk, uint64_t val)
+{
+     begin_timed(kReduce);
+     check(launch_masked_norm(d_state_, prec_, n_qubits(), msk, val, d_scratch_,
+                              d_scratch_ + kReduceScratchDoubles, stream_),
+           "masked_norm");
+     end_timed();
+     read_out(h_out_, 1);
+     return h_out_[0];
+}
+
+void Engine::normalize(double norm, uint64_t msk, uint64_t val)
+{
+     const double f = 1.0 / std::sqrt(norm);
+     begin_timed(kNormalize);
+     check(launch_normalize(d_state_, prec_, n_amps_, msk, val, f, stream_), "normalize");
+     end_timed();
+}
+
+std::vector<uint8_t> Engine::measure(const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     const uint64_t nb = std::min<uint64_t>(n_amps_, kMaxBlocks);
+     const uint64_t K = n_amps_ / nb;
+     begin_timed(kReduce);
+     check(launch_block_sums(d_state_, prec_, n_amps_, nb, d_blocks_, stream_), "block_sums");
+     end_timed();
+     check(hipMemcpyAsync(h_blocks_, d_blocks_, nb * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_),
+           "hipMemcpyAsync(blocks)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     std::partial_sum(h_blocks_, h_blocks_ + nb, h_blocks_);
+     const double u = dist_(rng_);
+     uint64_t b = 0;
+     for (; b < nb; ++b)
+          if (u <= h_blocks_[b]) break;
+     if (b == nb) {
+          std::ostringstream os;
+          os << "MeasureQubits(): Random number " << u << " > norm partial sum "
+             << h_blocks_[nb - 1];
+          throw std::runtime_error(os.str());
+     }
+     // In-block scan in index order, starting from the previous prefix
+     // (SimulatorMPI.cpp:943-961).
+     const size_t ab = amp_bytes(prec_);
+     if (K * ab > stage_bytes_) {
+          if (h_stage_) check(hipHostFree(h_stage_), "hipHostFree");
+          h_stage_ = nullptr;
+          check(hipHostMalloc(reinterpret_cast<void**>(&h_stage_), K * ab, hipHostMallocDefault),
+                "hipHostMalloc(stage)");
+          stage_bytes_ = K * ab;
+     }
+     check(hipMemcpyAsync(h_stage_, static_cast<char*>(d_state_) + b * K * ab, K * ab,
+                          hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync(block)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     double shift = b > 0 ? h_blocks_[b - 1] : 0.0;
+     uint64_t k = b * K;
+     for (uint64_t j = 0; j < K; ++j, ++k) {
+          double re, im;
+          if (prec_ == Prec::C128) {
+               re = reinterpret_cast<const double*>(h_stage_)[2 * j];
+               im = reinterpret_cast<const double*>(h_stage_)[2 * j + 1];
+          }
+          else {
+               re = reinterpret_cast<const float*>(h_stage_)[2 * j];
+               im = reinterpret_cast<const float*>(h_stage_)[2 * j + 1];
+          }
+          shift += re * re + im * im;
+          if (shift >= u) break;
+     }
+     std::vector<uint8_t> res(ids.size(), 0);
+     uint64_t msk = 0, val = 0;
+     for (size_t i = 0; i < ids.size(); ++i) {
+          const size_t p = find_sure(ids[i]);
+          msk |= uint64_t(1) << p;
+          if ((k >> p) & 1) {
+               val |= uint64_t(1) << p;
+               res[i] = 1;
+          }
+     }
+     const double norm = masked_norm(msk, val);
+     normalize(norm, msk, val);
+     return res;
+}
+
+double Engine::get_probability(const std::vector<uint8_t>& bits, const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     if (ids.size() != bits.size())
+          throw std::runtime_error("GetProbability(): ids.size() != bit_string.size()");
+     uint64_t msk, val;
+     mask_of(ids, bits, msk, val);
+     return masked_norm(msk, val);
+}
+
+cplx Engine::get_amplitude(const std::vector<uint8_t>& bits, const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     if (bits.size() != locals_.size() || bits.size() != ids.size())
+          throw std::runtime_error("GetAmplitude(): ids.size() != number of qubits");
+     uint64_t num = 0, chk = 0;
+     for (size_t i = 0; i < ids.size(); ++i) {
+          const size_t p = find_sure(ids[i]);
+          if (bits[i]) num |= uint64_t(1) << p;
+          chk |= uint64_t(1) << p;
+     }
+     const uint64_t full = locals_.empty() ? 0 : ((uint64_t(1) << locals_.size()) - 1);
+     if (chk != full)
+          throw std::runtime_error(
+              "GetAmplitude(): the second argument must be a permutation of all allocated "
+              "qubits.");
+     const size_t ab = amp_bytes(prec_);
+     char buf[16];
+     check(hipMemcpyAsync(buf, static_cast<char*>(d_state_) + num * ab, ab, hipMemcpyDeviceToHost,
+                          stream_),
+           "hipMemcpyAsync(amplitude)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     if (prec_ == Prec::C128) {
+          double v[2];
+          std::memcpy(v, buf, 16);
+          return cplx(v[0], v[1]);
+     }
+     float v[2];
+     std::memcpy(v, buf, 8);
+     return cplx(v[0], v[1]);
+}
+
+void Engine::collapse(const std::vector<int64_t>& ids, const std::vector<uint8_t>& values)
+{
+     DeviceScope ds(device_);
+     if (ids.size() != values.size())
+          throw std::runtime_error("collapseWaveFunction(): ids.size() != values.size()");
+     uint64_t msk, val;
+     mask_of(ids, values, msk, val);
+     const double norm = masked_norm(msk, val);
+     if (norm < 1.e-12)
+          throw std::runtime_error("collapseWaveFunction(): Invalid collapse! Probability is ~0.");
+     normalize(norm, msk, val);
+}
+
+double Engine::entropy()
+{
+     DeviceScope ds(device_);
+     begin_timed(kReduce);
+     check(launch_entropy(d_state_, prec_, n_amps_, d_scratch_, d_scratch_ + kReduceScratchDoubles,
+                          stream_),
+           "entropy");
+     end_timed();
+     read_out(h_out_, 1);
+     return -h_out_[0];
+}
+
+// ---------------------------------------------------------------------------
+// State transfer
+// ---------------------------------------------------------------------------
+void Engine::get_state_c128(std::complex<double>* out)
+{
+     DeviceScope ds(device_);
+     if (prec_ == Prec::C128) {
+          check(hipMemcpyAsync(out, d_state_, n_amps_ * 16, hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync(state)");
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          return;
+     }
+     std::vector<std::complex<float>> tmp(n_amps_);
+     check(hipMemcpyAsync(tmp.data(), d_state_, n_amps_ * 8, hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync(state)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     for (uint64_t i = 0; i < n_amps_; ++i) out[i] = std::complex<double>(tmp[i]);
+}
+
+void Engine::set_state_c128(const std::complex<double>* in, uint64_t n)
+{
+     DeviceScope ds(device_);
+     if (n != n_amps_)
+          throw std::runtime_error("set_wavefunction(): expected " + std::to_string(n_amps_)
+                                   + " amplitudes, got " + std::to_string(n));
+     if (prec_ == Prec::C128) {
+          check(hipMemcpyAsync(d_state_, in, n * 16, hipMemcpyHostToDevice, stream_),
+                "hipMemcpyAsync(state)");
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          return;
+     }
+     std::vector<std::complex<float>> tmp(n);
+     for (uint64_t i = 0; i < n; ++i) tmp[i] = std::complex<float>(in[i]);
+     check(hipMemcpyAsync(d_state_, tmp.data(), n * 8, hipMemcpyHostToDevice, stream_),
+           "hipMemcpyAsync(state)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------------------
+// Pauli sums
+// ---------------------------------------------------------------------------
+namespace {
+inline cplx ipow(int e)
+{
+     switch (e & 3) {
+          case 0: return cplx(1, 0);
+          case 1: return cplx(0, 1);
+          case 2: return cplx(-1, 0);
+          default: return cplx(0, -1);
+     }
+}
+}  // namespace
+
+cplx Engine::expectation(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                         const std::vector<cplx>& coeff, const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     if (xm.size() != zm.size() || xm.size() != coeff.size())
+          throw std::runtime_error("expectation(): term arrays differ in length");
+     std::vector<uint32_t> pos(ids.size());
+     for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
+     auto to_bits = [&](uint64_t m) {
+          uint64_t r = 0;
+          for (size_t i = 0; i < 64; ++i)
+               if ((m >> i) & 1) {
+                    if (i >= ids.size())
+                         throw std::runtime_error(
+                             "qubit_operator acts on more qubits than contained in the qureg.");
+                    r |= uint64_t(1) << pos[i];
+               }
+          return r;
+     };
+     cplx total(0.0, 0.0);
+     for (size_t t0 = 0; t0 < xm.size(); t0 += kMaxOuts / 2) {
+          const size_t t1 = std::min(xm.size(), t0 + kMaxOuts / 2);
+          for (size_t t = t0; t < t1; ++t) {
+               begin_timed(kPauli);
+               check(launch_pauli_expectation(d_state_, prec_, n_amps_, to_bits(xm[t]),
+                                              to_bits(zm[t]), d_scratch_,
+                                              d_scratch_ + kReduceScratchDoubles + 2 * (t - t0),
+                                              stream_),
+                     "pauli_expectation");
+               end_timed();
+          }
+          read_out(h_out_, 2 * (t1 - t0));
+          for (size_t t = t0; t < t1; ++t) {
+               const cplx s(h_out_[2 * (t - t0)], h_out_[2 * (t - t0) + 1]);
+               total += coeff[t] * ipow(__builtin_popcountll(xm[t] & zm[t])) * s;
+          }
+     }
+     return total;
+}
+
+void Engine::apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                             const std::vector<cplx>& coeff, const std::vector<int64_t>& ids)
+{
+     DeviceScope ds(device_);
+     if (xm.size() != zm.size() || xm.size() != coeff.size())
+          throw std::runtime_error("apply_qubit_operator(): term arrays differ in length");
+     std::vector<uint32_t> pos(ids.size());
+     for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
+     auto to_bits = [&](uint64_t m) {
+          uint64_t r = 0;
+          for (size_t i = 0; i < 64; ++i)
+               if ((m >> i) & 1) {
+                    if (i >= ids.size())
+                         throw std::runtime_error(
+                             "qubit_operator acts on more qubits than contained in the qureg.");
+                    r |= uint64_t(1) << pos[i];
+               }
+          return r;
+     };
+     std::vector<uint64_t> X(xm.size()), Z(xm.size());
+     for (size_t t = 0; t < xm.size(); ++t) {
+          X[t] = to_bits(xm[t]);
+          Z[t] = to_bits(zm[t]);
+     }
+     ensure_tmp(n_amps_);
+     begin_timed(kFillCopy);
+     check(launch_fill(d_tmp_, prec_, n_amps_, 0.0, 0.0, stream_), "fill");
+     end_timed();
+     for (size_t t = 0; t < xm.size(); ++t) {
+          const cplx c = coeff[t] * ipow(__builtin_popcountll(xm[t] & zm[t]));
+          begin_timed(kPauli);
+          check(launch_pauli_accumulate(d_tmp_, d_state_, prec_, n_amps_, X[t], Z[t], c.real(),
+                                        c.imag(), stream_),
+                "pauli_accumulate");
+          end_timed();
+     }
+     swap_tmp();
+}
+
+}  // namespace hq

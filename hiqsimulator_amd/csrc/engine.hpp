@@ -1,0 +1,163 @@
+// Single-GPU state-vector engine: the host side of the hot path.
+//
+// Mirrors the behaviour of the reference's SimulatorMPI at world size 1
+// (src/simulator-mpi/SimulatorMPI.{hpp,cpp}): qubit bookkeeping, gate
+// canonicalisation with control folding, fusion, one fused HBM pass per
+// run(), measurement with the reference's block-CDF sampling and
+// std::mt19937 stream, probabilities, amplitudes, collapse, deallocation.
+// Errors are thrown as std::runtime_error with the reference's messages.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "fusion.hpp"
+#include "kernels.hpp"
+
+namespace hq {
+
+// Timed kernel classes (for the HIP-event timing mode).
+enum KernelClass : int {
+     kDense1 = 0,
+     kDense2,
+     kDense3,
+     kDense4,
+     kDense5,
+     kDiag,
+     kScale,
+     kPermute,
+     kReduce,
+     kNormalize,
+     kRemoveBit,
+     kFillCopy,
+     kPauli,
+     kNumKernelClasses
+};
+
+class Engine {
+public:
+     Engine(uint64_t seed, int64_t max_local, int64_t max_cluster, Prec prec, int device);
+     ~Engine();
+     Engine(const Engine&) = delete;
+     Engine& operator=(const Engine&) = delete;
+
+     // SimulatorMPI.cpp:168-251.
+     void allocate_qubit(int64_t id);
+     void allocate_qureg(const std::vector<int64_t>& ids, cplx init);
+     // SimulatorMPI.cpp:278-322, 369-426.
+     void deallocate_qubit(int64_t id);
+     // SimulatorMPI.cpp:710-815.
+     void apply_gate(Mat m, std::vector<int64_t> ids, const std::vector<int64_t>& ctrls);
+     // SimulatorMPI.cpp:441-541.
+     void run();
+     // SimulatorMPI.cpp:897-1008.
+     std::vector<uint8_t> measure(const std::vector<int64_t>& ids);
+     // SimulatorMPI.cpp:561-600.
+     double get_probability(const std::vector<uint8_t>& bits, const std::vector<int64_t>& ids);
+     // SimulatorMPI.cpp:602-650.
+     cplx get_amplitude(const std::vector<uint8_t>& bits, const std::vector<int64_t>& ids);
+     // SimulatorMPI.cpp:1010-1058.
+     void collapse(const std::vector<int64_t>& ids, const std::vector<uint8_t>& values);
+     // SimulatorMPI.cpp:681-694.
+     double entropy();
+
+     // SimulatorMPI.cpp:652-667 (relabel only, no data movement).
+     const std::vector<int64_t>& qubit_ids() const { return locals_; }
+     void set_qubits_perm(const std::vector<int64_t>& ids);
+     // SimulatorMPI.cpp:1060-1138: at one device there are no global qubits.
+     void swap_qubits(const std::vector<int64_t>& pairs);
+     // Moves data so that ids[p] becomes the qubit at bit p (the in-HBM
+     // replacement of the reference's local<->global qubit exchange).
+     void permute_qubits(const std::vector<int64_t>& ids);
+
+     // State transfer (cheat_local, SimulatorMPI.cpp:543-559; set_wavefunction,
+     // _simulator_mpi.py:281-306).
+     uint64_t state_size() const { return n_amps_; }
+     Prec precision() const { return prec_; }
+     void get_state_c128(std::complex<double>* out);
+     void set_state_c128(const std::complex<double>* in, uint64_t n);
+
+     // Pauli sums; mask bit i refers to ids[i].  Y is encoded as X and Z.
+     cplx expectation(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                      const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
+     void apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                          const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
+
+     // One immediate pass at explicit bit positions (bypasses fusion).
+     void apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
+                       bool diag);
+
+     void synchronize();
+     void set_timing(bool on);
+     void kernel_time(int cls, uint64_t& count, double& total_ms);
+     hipStream_t stream() const { return stream_; }
+     int device() const { return device_; }
+
+private:
+     struct TimedLaunch {
+          int cls;
+          hipEvent_t start, stop;
+     };
+
+     size_t find(int64_t id) const;
+     size_t find_sure(int64_t id) const;
+     void ensure_state(uint64_t n_amps);
+     void ensure_tmp(uint64_t n_amps);
+     void swap_tmp();
+     const void* upload_matrix(const Mat& m);
+     void dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
+                        bool diag);
+     double masked_norm(uint64_t msk, uint64_t val);
+     void mask_of(const std::vector<int64_t>& ids, const std::vector<uint8_t>& bits,
+                  uint64_t& msk, uint64_t& val) const;
+     void normalize(double norm, uint64_t msk, uint64_t val);
+     int n_qubits() const { return static_cast<int>(locals_.size()); }
+     void begin_timed(int cls);
+     void end_timed();
+     void read_out(double* host, size_t count);
+
+     Prec prec_;
+     int device_;
+     size_t max_local_, max_cluster_, min_local_;
+     std::vector<int64_t> locals_;
+     Fusion fusion_;
+     std::mt19937 rng_;
+     std::uniform_real_distribution<double> dist_{0.0, 1.0};
+
+     hipStream_t stream_ = nullptr;
+     void* d_state_ = nullptr;
+     uint64_t cap_state_ = 0;  // amplitudes
+     void* d_tmp_ = nullptr;
+     uint64_t cap_tmp_ = 0;
+     uint64_t n_amps_ = 1;
+
+     // Fused-matrix upload ring: pinned host slots copied into device slots.
+     static constexpr int kRing = 64;
+     static constexpr size_t kSlotBytes = 32 * 32 * 16;
+     char* h_ring_ = nullptr;
+     char* d_ring_ = nullptr;
+     hipEvent_t ring_ev_[kRing] = {};
+     bool ring_used_[kRing] = {};
+     int ring_next_ = 0;
+
+     double* d_scratch_ = nullptr;  // reductions + outputs
+     double* h_out_ = nullptr;      // pinned, 16 doubles
+     double* d_blocks_ = nullptr;   // measurement block sums (2^15)
+     double* h_blocks_ = nullptr;   // pinned
+     char* h_stage_ = nullptr;      // pinned staging for block scans
+     size_t stage_bytes_ = 0;
+
+     bool timing_ = false;
+     std::vector<TimedLaunch> pending_;
+     std::vector<hipEvent_t> free_events_;
+     uint64_t t_count_[kNumKernelClasses] = {};
+     double t_ms_[kNumKernelClasses] = {};
+};
+
+}  // namespace hq

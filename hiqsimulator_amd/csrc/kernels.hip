@@ -1,0 +1,719 @@
+// gfx950 (CDNA4) kernels of the state-vector hot path.
+//
+// Every kernel is one streaming pass over the state in HBM; the bound is HBM
+// bandwidth (see launch_* comments for the algorithmic bytes).  Threads map to
+// "groups" (the 2^k amplitudes one fused matrix mixes) so that consecutive
+// lanes of a wave take consecutive group bases: every global load and store of
+// a wave then covers contiguous 16 B (complex128) or 8 B (complex64) slots
+// whenever the low index bits are not targets.  The fused matrix is uniform
+// across the grid and is read with scalar loads into SGPRs, which v_fma_f64 /
+// v_fma_f32 take directly as an operand.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace hq {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint64_t kMaxGrid = 1ull << 20;
+
+template <typename R>
+struct C2;
+template <>
+struct C2<double> {
+     using T = double2;
+};
+template <>
+struct C2<float> {
+     using T = float2;
+};
+
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t t, const InsertSet& s)
+{
+     for (int i = 0; i < s.n; ++i) {
+          const uint64_t lo = t & ((1ull << s.p[i]) - 1ull);
+          t = ((t ^ lo) << 1) | lo;
+     }
+     return t;
+}
+
+inline unsigned grid_for(uint64_t work_items, uint64_t per_block)
+{
+     uint64_t g = (work_items + per_block - 1) / per_block;
+     if (g == 0) g = 1;
+     if (g > kMaxGrid) g = kMaxGrid;
+     return static_cast<unsigned>(g);
+}
+
+// ---------------------------------------------------------------------------
+// Dense fused k-qubit pass.  Restates kernels/intrin/kernel{1..5}.hpp
+// (kernel_core + kernel_compute): for each base I with target bits 0 and
+// control bits 1, v_j = psi[I + off_j], psi[I + off_i] = sum_j m[i][j] v_j.
+// GPT groups per thread are loaded before any is computed so that
+// GPT * 2^K independent loads are in flight per lane.
+// ---------------------------------------------------------------------------
+template <int K, typename R, int GPT>
+__global__ void __launch_bounds__(kBlock)
+    k_dense(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
+            const DenseDesc d, const uint64_t ngroups)
+{
+     using V = typename C2<R>::T;
+     constexpr int D = 1 << K;
+     uint64_t off[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          uint64_t o = 0;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) o += 1ull << d.pos[l];
+          off[j] = o;
+     }
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * (kBlock * GPT);
+     for (uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * (kBlock * GPT) + threadIdx.x;
+          t0 < ngroups; t0 += stride) {
+          uint64_t I[GPT];
+          V v[GPT][D];
+#pragma unroll
+          for (int g = 0; g < GPT; ++g) {
+               const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
+               I[g] = insert_zeros(t < ngroups ? t : 0, d.ins) | d.ctrlmask;
+               if (t < ngroups) {
+#pragma unroll
+                    for (int j = 0; j < D; ++j) v[g][j] = psi[I[g] + off[j]];
+               }
+          }
+#pragma unroll
+          for (int g = 0; g < GPT; ++g) {
+               const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
+               if (t >= ngroups) continue;
+#pragma unroll
+               for (int i = 0; i < D; ++i) {
+                    R re = 0, im = 0;
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                         const V m = mat[i * D + j];
+                         re = fma(m.x, v[g][j].x, re);
+                         re = fma(-m.y, v[g][j].y, re);
+                         im = fma(m.x, v[g][j].y, im);
+                         im = fma(m.y, v[g][j].x, im);
+                    }
+                    V o;
+                    o.x = re;
+                    o.y = im;
+                    psi[I[g] + off[i]] = o;
+               }
+          }
+     }
+}
+
+template <typename R>
+struct DiagVals {
+     R v[64];  // 32 complex values, interleaved
+};
+
+// Diagonal fused pass: kernels/intrin/kernels_diag.hpp:34-144.  Threads
+// enumerate the indices whose control bits are 1 (zero bits inserted at the
+// control positions, ctrlmask OR-ed in); d(i) gathers the target bits.
+template <int K, typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_diag(typename C2<R>::T* __restrict__ psi, const DiagVals<R> dv, const InsertSet cins,
+           const uint64_t ctrlmask, const uint32_t pos0, const uint32_t pos1, const uint32_t pos2,
+           const uint32_t pos3, const uint32_t pos4, const uint64_t count)
+{
+     using V = typename C2<R>::T;
+     constexpr int E = 4;
+     const uint32_t pos[5] = {pos0, pos1, pos2, pos3, pos4};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * (kBlock * E);
+     for (uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * (kBlock * E) + threadIdx.x; t0 < count;
+          t0 += stride) {
+#pragma unroll
+          for (int e = 0; e < E; ++e) {
+               const uint64_t t = t0 + static_cast<uint64_t>(e) * kBlock;
+               if (t >= count) break;
+               const uint64_t i = insert_zeros(t, cins) | ctrlmask;
+               uint32_t di = 0;
+#pragma unroll
+               for (int l = 0; l < K; ++l) di |= static_cast<uint32_t>((i >> pos[l]) & 1ull) << l;
+               const V a = psi[i];
+               const R mr = dv.v[2 * di], mi = dv.v[2 * di + 1];
+               V o;
+               o.x = fma(a.x, mr, -a.y * mi);
+               o.y = fma(a.x, mi, a.y * mr);
+               psi[i] = o;
+          }
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_scale(typename C2<R>::T* __restrict__ psi, const R fr, const R fi, const uint64_t n)
+{
+     using V = typename C2<R>::T;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const V a = psi[i];
+          V o;
+          o.x = fma(a.x, fr, -a.y * fi);
+          o.y = fma(a.x, fi, a.y * fr);
+          psi[i] = o;
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_fill(typename C2<R>::T* __restrict__ psi, const R vr, const R vi, const uint64_t n)
+{
+     using V = typename C2<R>::T;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     V o;
+     o.x = vr;
+     o.y = vi;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+          psi[i] = o;
+}
+
+// Plain streaming copy in 16-byte words.
+__global__ void __launch_bounds__(kBlock)
+    k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, const uint64_t nwords)
+{
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < nwords;
+          i += stride)
+          dst[i] = src[i];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_normalize(typename C2<R>::T* __restrict__ psi, const uint64_t n, const uint64_t msk,
+                const uint64_t val, const R f)
+{
+     using V = typename C2<R>::T;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          V a = psi[i];
+          if ((i & msk) != val) {
+               a.x = 0;
+               a.y = 0;
+          }
+          else {
+               a.x *= f;
+               a.y *= f;
+          }
+          psi[i] = a;
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_remove_bit(typename C2<R>::T* __restrict__ dst, const typename C2<R>::T* __restrict__ src,
+                 const uint64_t n_dst, const int pos, const uint64_t keep)
+{
+     const uint64_t lo_mask = (1ull << pos) - 1ull;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < n_dst;
+          j += stride) {
+          const uint64_t lo = j & lo_mask;
+          const uint64_t i = ((j ^ lo) << 1) | lo | (keep << pos);
+          dst[j] = src[i];
+     }
+}
+
+// Qubit-index permutation: dst[j] = src[i(j)], bit p of j = bit src_pos[p] of
+// i.  i(j) is assembled from five 8-bit lookup tables built in LDS.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_permute(typename C2<R>::T* __restrict__ dst, const typename C2<R>::T* __restrict__ src,
+              const uint64_t n, const int nq, const InsertSet sp)
+{
+     __shared__ uint64_t tab[5][256];
+     for (int c = 0; c < 5; ++c) {
+          const int v = threadIdx.x;
+          uint64_t r = 0;
+          for (int b = 0; b < 8; ++b) {
+               const int p = 8 * c + b;
+               if (p < nq && ((v >> b) & 1)) r |= 1ull << sp.p[p];
+          }
+          tab[c][v] = r;
+     }
+     __syncthreads();
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride) {
+          const uint64_t i = tab[0][j & 255] | tab[1][(j >> 8) & 255] | tab[2][(j >> 16) & 255]
+                             | tab[3][(j >> 24) & 255] | tab[4][(j >> 32) & 255];
+          dst[j] = src[i];
+     }
+}
+
+// dst[j] += c * (-1)^{popcount((j ^ x) & z)} * src[j ^ x].
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pauli_acc(typename C2<R>::T* __restrict__ dst, const typename C2<R>::T* __restrict__ src,
+                const uint64_t n, const uint64_t x, const uint64_t z, const R cr, const R ci)
+{
+     using V = typename C2<R>::T;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride) {
+          const uint64_t i = j ^ x;
+          const V a = src[i];
+          const R s = (__popcll(i & z) & 1) ? R(-1) : R(1);
+          V o = dst[j];
+          o.x += s * (cr * a.x - ci * a.y);
+          o.y += s * (cr * a.y + ci * a.x);
+          dst[j] = o;
+     }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: fixed grid (<= kReduceBlocks blocks), per-thread FP64
+// accumulation in a grid-stride loop, wave shuffle + LDS tree inside a block,
+// one partial per block, then a single-block final tree.  Deterministic for a
+// given size.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+     return v;
+}
+
+template <int NOUT>
+__device__ __forceinline__ void block_store(double (&acc)[NOUT], double* partials, int nblocks)
+{
+     __shared__ double red[NOUT][kBlock / 64];
+     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+     for (int o = 0; o < NOUT; ++o) {
+          const double s = wave_sum(acc[o]);
+          if (lane == 0) red[o][w] = s;
+     }
+     __syncthreads();
+     if (threadIdx.x == 0) {
+#pragma unroll
+          for (int o = 0; o < NOUT; ++o) {
+               double s = 0;
+               for (int i = 0; i < kBlock / 64; ++i) s += red[o][i];
+               partials[o * nblocks + blockIdx.x] = s;
+          }
+     }
+}
+
+template <int NOUT>
+__global__ void __launch_bounds__(kBlock)
+    k_reduce_final(const double* __restrict__ partials, const int nblocks, double* __restrict__ out)
+{
+     double acc[NOUT];
+#pragma unroll
+     for (int o = 0; o < NOUT; ++o) {
+          double s = 0;
+          for (int i = threadIdx.x; i < nblocks; i += kBlock) s += partials[o * nblocks + i];
+          acc[o] = s;
+     }
+     __shared__ double red[NOUT][kBlock / 64];
+     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+     for (int o = 0; o < NOUT; ++o) {
+          const double s = wave_sum(acc[o]);
+          if (lane == 0) red[o][w] = s;
+     }
+     __syncthreads();
+     if (threadIdx.x == 0) {
+#pragma unroll
+          for (int o = 0; o < NOUT; ++o) {
+               double s = 0;
+               for (int i = 0; i < kBlock / 64; ++i) s += red[o][i];
+               out[o] = s;
+          }
+     }
+}
+
+template <typename R>
+__device__ __forceinline__ double norm2(const typename C2<R>::T a)
+{
+     const double x = a.x, y = a.y;
+     return x * x + y * y;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_masked_norm(const typename C2<R>::T* __restrict__ psi, const uint64_t count,
+                  const InsertSet ins, const uint64_t val, double* __restrict__ partials)
+{
+     double acc[1] = {0.0};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; t < count;
+          t += stride)
+          acc[0] += norm2<R>(psi[insert_zeros(t, ins) | val]);
+     block_store<1>(acc, partials, gridDim.x);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_two_bucket(const typename C2<R>::T* __restrict__ psi, const uint64_t n, const int pos,
+                 double* __restrict__ partials)
+{
+     double acc[2] = {0.0, 0.0};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const double p = norm2<R>(psi[i]);
+          if ((i >> pos) & 1) acc[1] += p;
+          else acc[0] += p;
+     }
+     block_store<2>(acc, partials, gridDim.x);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_entropy(const typename C2<R>::T* __restrict__ psi, const uint64_t n,
+              double* __restrict__ partials)
+{
+     double acc[1] = {0.0};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const double a = hypot(static_cast<double>(psi[i].x), static_cast<double>(psi[i].y));
+          const double pr = a * a;
+          if (pr > 0) acc[0] += pr * log2(pr);
+     }
+     block_store<1>(acc, partials, gridDim.x);
+}
+
+// sum_i conj(psi[i ^ x]) psi[i] (-1)^{popcount(i & z)}; the uniform factor
+// i^{popcount(x & z)} is applied by the host.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_pauli_exp(const typename C2<R>::T* __restrict__ psi, const uint64_t n, const uint64_t x,
+                const uint64_t z, double* __restrict__ partials)
+{
+     double acc[2] = {0.0, 0.0};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const double ar = psi[i].x, ai = psi[i].y;
+          const double br = psi[i ^ x].x, bi = psi[i ^ x].y;
+          const double s = (__popcll(i & z) & 1) ? -1.0 : 1.0;
+          acc[0] += s * (br * ar + bi * ai);
+          acc[1] += s * (br * ai - bi * ar);
+     }
+     block_store<2>(acc, partials, gridDim.x);
+}
+
+// Measurement block sums (SimulatorMPI.cpp:817-839): one workgroup per
+// contiguous block when the block is long, one thread per block otherwise.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_block_sums_wg(const typename C2<R>::T* __restrict__ psi, const uint64_t K,
+                    double* __restrict__ out)
+{
+     const uint64_t b = blockIdx.x;
+     double acc[1] = {0.0};
+     for (uint64_t j = threadIdx.x; j < K; j += kBlock) acc[0] += norm2<R>(psi[b * K + j]);
+     __shared__ double red[kBlock / 64];
+     const double s = wave_sum(acc[0]);
+     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+     __syncthreads();
+     if (threadIdx.x == 0) {
+          double r = 0;
+          for (int i = 0; i < kBlock / 64; ++i) r += red[i];
+          out[b] = r;
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_block_sums_seq(const typename C2<R>::T* __restrict__ psi, const uint64_t K, const uint64_t nb,
+                     double* __restrict__ out)
+{
+     const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+     if (b >= nb) return;
+     double s = 0;
+     for (uint64_t j = 0; j < K; ++j) s += norm2<R>(psi[b * K + j]);
+     out[b] = s;
+}
+
+template <typename R>
+using VT = typename C2<R>::T;
+
+template <int K, typename R>
+hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStream_t s)
+{
+     constexpr int GPT = K >= 4 ? 1 : (K == 3 ? 2 : 4);
+     const int nins = d.ins.n;
+     if (nins > n) return hipErrorInvalidValue;
+     const uint64_t ngroups = 1ull << (n - nins);
+     const unsigned grid = grid_for(ngroups, static_cast<uint64_t>(kBlock) * GPT);
+     hipLaunchKernelGGL((k_dense<K, R, GPT>), dim3(grid), dim3(kBlock), 0, s,
+                        static_cast<VT<R>*>(psi), static_cast<const VT<R>*>(mat), d, ngroups);
+     return hipGetLastError();
+}
+
+template <typename R>
+hipError_t dense_dispatch(void* psi, int n, const void* mat, const DenseDesc& d, hipStream_t s)
+{
+     switch (d.k) {
+          case 1: return dense_k<1, R>(psi, n, mat, d, s);
+          case 2: return dense_k<2, R>(psi, n, mat, d, s);
+          case 3: return dense_k<3, R>(psi, n, mat, d, s);
+          case 4: return dense_k<4, R>(psi, n, mat, d, s);
+          case 5: return dense_k<5, R>(psi, n, mat, d, s);
+          default: return hipErrorInvalidValue;
+     }
+}
+
+InsertSet ctrl_insert_set(uint64_t ctrlmask)
+{
+     InsertSet s{};
+     for (int p = 0; p < 64 && s.n < 63; ++p)
+          if ((ctrlmask >> p) & 1) s.p[s.n++] = static_cast<uint8_t>(p);
+     return s;
+}
+
+template <int K, typename R>
+hipError_t diag_k(void* psi, int n, const double* dre, const DenseDesc& d, hipStream_t s)
+{
+     DiagVals<R> dv{};
+     for (int i = 0; i < (2 << K); ++i) dv.v[i] = static_cast<R>(dre[i]);
+     const InsertSet cins = ctrl_insert_set(d.ctrlmask);
+     if (cins.n > n) return hipErrorInvalidValue;
+     const uint64_t count = 1ull << (n - cins.n);
+     const unsigned grid = grid_for(count, static_cast<uint64_t>(kBlock) * 4);
+     hipLaunchKernelGGL((k_diag<K, R>), dim3(grid), dim3(kBlock), 0, s, static_cast<VT<R>*>(psi),
+                        dv, cins, d.ctrlmask, d.pos[0], d.pos[1], d.pos[2], d.pos[3], d.pos[4],
+                        count);
+     return hipGetLastError();
+}
+
+template <typename R>
+hipError_t diag_dispatch(void* psi, int n, const double* dre, const DenseDesc& d, hipStream_t s)
+{
+     switch (d.k) {
+          case 1: return diag_k<1, R>(psi, n, dre, d, s);
+          case 2: return diag_k<2, R>(psi, n, dre, d, s);
+          case 3: return diag_k<3, R>(psi, n, dre, d, s);
+          case 4: return diag_k<4, R>(psi, n, dre, d, s);
+          case 5: return diag_k<5, R>(psi, n, dre, d, s);
+          default: return hipErrorInvalidValue;
+     }
+}
+
+template <int NOUT, class Launch>
+hipError_t reduce_launch(uint64_t work, double* scratch, double* out, hipStream_t s, Launch&& l)
+{
+     uint64_t g = (work + kBlock - 1) / kBlock;
+     if (g < 1) g = 1;
+     if (g > static_cast<uint64_t>(kReduceBlocks)) g = kReduceBlocks;
+     const int grid = static_cast<int>(g);
+     l(grid);
+     hipError_t e = hipGetLastError();
+     if (e != hipSuccess) return e;
+     hipLaunchKernelGGL((k_reduce_final<NOUT>), dim3(1), dim3(kBlock), 0, s, scratch, grid, out);
+     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
+                        hipStream_t s)
+{
+     return prec == Prec::C128 ? dense_dispatch<double>(psi, n_qubits, mat, d, s)
+                               : dense_dispatch<float>(psi, n_qubits, mat, d, s);
+}
+
+hipError_t launch_diag(void* psi, Prec prec, int n_qubits, const double* diag_re_im,
+                       const DenseDesc& d, hipStream_t s)
+{
+     return prec == Prec::C128 ? diag_dispatch<double>(psi, n_qubits, diag_re_im, d, s)
+                               : diag_dispatch<float>(psi, n_qubits, diag_re_im, d, s);
+}
+
+hipError_t launch_scale(void* psi, Prec prec, uint64_t n, double fre, double fim, hipStream_t s)
+{
+     const unsigned grid = grid_for(n, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_scale<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(psi), fre, fim, n);
+     else
+          hipLaunchKernelGGL((k_scale<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(psi), static_cast<float>(fre),
+                             static_cast<float>(fim), n);
+     return hipGetLastError();
+}
+
+hipError_t launch_fill(void* psi, Prec prec, uint64_t n, double vre, double vim, hipStream_t s)
+{
+     const unsigned grid = grid_for(n, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_fill<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(psi), vre, vim, n);
+     else
+          hipLaunchKernelGGL((k_fill<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(psi), static_cast<float>(vre),
+                             static_cast<float>(vim), n);
+     return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, Prec prec, uint64_t n, hipStream_t s)
+{
+     const uint64_t bytes = n * amp_bytes(prec);
+     const uint64_t words = bytes / 16;
+     if (words > 0) {
+          const unsigned grid = grid_for(words, kBlock * 4);
+          hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, static_cast<uint4*>(dst),
+                             static_cast<const uint4*>(src), words);
+          hipError_t e = hipGetLastError();
+          if (e != hipSuccess) return e;
+     }
+     const uint64_t tail = bytes - words * 16;
+     if (tail)
+          return hipMemcpyAsync(static_cast<char*>(dst) + words * 16,
+                                static_cast<const char*>(src) + words * 16, tail,
+                                hipMemcpyDeviceToDevice, s);
+     return hipSuccess;
+}
+
+hipError_t launch_normalize(void* psi, Prec prec, uint64_t n, uint64_t msk, uint64_t val, double f,
+                            hipStream_t s)
+{
+     const unsigned grid = grid_for(n, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_normalize<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(psi), n, msk, val, f);
+     else
+          hipLaunchKernelGGL((k_normalize<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(psi), n, msk, val, static_cast<float>(f));
+     return hipGetLastError();
+}
+
+hipError_t launch_remove_bit(void* dst, const void* src, Prec prec, uint64_t n_src, int pos,
+                             int keep, hipStream_t s)
+{
+     const uint64_t n_dst = n_src / 2;
+     const unsigned grid = grid_for(n_dst, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_remove_bit<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(dst), static_cast<const double2*>(src), n_dst,
+                             pos, static_cast<uint64_t>(keep));
+     else
+          hipLaunchKernelGGL((k_remove_bit<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(dst), static_cast<const float2*>(src), n_dst, pos,
+                             static_cast<uint64_t>(keep));
+     return hipGetLastError();
+}
+
+hipError_t launch_permute(void* dst, const void* src, Prec prec, int n_qubits,
+                          const uint8_t* src_pos, hipStream_t s)
+{
+     if (n_qubits > 40) return hipErrorInvalidValue;
+     InsertSet sp{};
+     sp.n = n_qubits;
+     for (int p = 0; p < n_qubits; ++p) sp.p[p] = src_pos[p];
+     const uint64_t n = 1ull << n_qubits;
+     const unsigned grid = grid_for(n, kBlock * 8);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_permute<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(dst), static_cast<const double2*>(src), n,
+                             n_qubits, sp);
+     else
+          hipLaunchKernelGGL((k_permute<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(dst), static_cast<const float2*>(src), n,
+                             n_qubits, sp);
+     return hipGetLastError();
+}
+
+hipError_t launch_pauli_accumulate(void* dst, const void* src, Prec prec, uint64_t n,
+                                   uint64_t xmask, uint64_t zmask, double cre, double cim,
+                                   hipStream_t s)
+{
+     const unsigned grid = grid_for(n, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_pauli_acc<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(dst), static_cast<const double2*>(src), n,
+                             xmask, zmask, cre, cim);
+     else
+          hipLaunchKernelGGL((k_pauli_acc<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(dst), static_cast<const float2*>(src), n, xmask,
+                             zmask, static_cast<float>(cre), static_cast<float>(cim));
+     return hipGetLastError();
+}
+
+hipError_t launch_masked_norm(const void* psi, Prec prec, int n_qubits, uint64_t msk, uint64_t val,
+                              double* scratch, double* out, hipStream_t s)
+{
+     const InsertSet ins = ctrl_insert_set(msk);
+     if (ins.n > n_qubits) return hipErrorInvalidValue;
+     const uint64_t count = 1ull << (n_qubits - ins.n);
+     return reduce_launch<1>(count, scratch, out, s, [&](int grid) {
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_masked_norm<double>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const double2*>(psi), count, ins, val, scratch);
+          else
+               hipLaunchKernelGGL((k_masked_norm<float>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const float2*>(psi), count, ins, val, scratch);
+     });
+}
+
+hipError_t launch_two_bucket(const void* psi, Prec prec, int n_qubits, int pos, double* scratch,
+                             double* out, hipStream_t s)
+{
+     const uint64_t n = 1ull << n_qubits;
+     return reduce_launch<2>(n, scratch, out, s, [&](int grid) {
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_two_bucket<double>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const double2*>(psi), n, pos, scratch);
+          else
+               hipLaunchKernelGGL((k_two_bucket<float>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const float2*>(psi), n, pos, scratch);
+     });
+}
+
+hipError_t launch_entropy(const void* psi, Prec prec, uint64_t n, double* scratch, double* out,
+                          hipStream_t s)
+{
+     return reduce_launch<1>(n, scratch, out, s, [&](int grid) {
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_entropy<double>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const double2*>(psi), n, scratch);
+          else
+               hipLaunchKernelGGL((k_entropy<float>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const float2*>(psi), n, scratch);
+     });
+}
+
+hipError_t launch_pauli_expectation(const void* psi, Prec prec, uint64_t n, uint64_t xmask,
+                                    uint64_t zmask, double* scratch, double* out, hipStream_t s)
+{
+     return reduce_launch<2>(n, scratch, out, s, [&](int grid) {
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_pauli_exp<double>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const double2*>(psi), n, xmask, zmask, scratch);
+          else
+               hipLaunchKernelGGL((k_pauli_exp<float>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const float2*>(psi), n, xmask, zmask, scratch);
+     });
+}
+
+hipError_t launch_block_sums(const void* psi, Prec prec, uint64_t n, uint64_t nb, double* out,
+                             hipStream_t s)
+{
+     if (nb == 0 || n % nb != 0) return hipErrorInvalidValue;
+     const uint64_t K = n / nb;
+     if (K >= kBlock) {
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_block_sums_wg<double>), dim3(static_cast<unsigned>(nb)),
+                                  dim3(kBlock), 0, s, static_cast<const double2*>(psi), K, out);
+          else
+               hipLaunchKernelGGL((k_block_sums_wg<float>), dim3(static_cast<unsigned>(nb)),
+                                  dim3(kBlock), 0, s, static_cast<const float2*>(psi), K, out);
+     }
+     else {
+          const unsigned grid = static_cast<unsigned>((nb + kBlock - 1) / kBlock);
+          if (prec == Prec::C128)
+               hipLaunchKernelGGL((k_block_sums_seq<double>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const double2*>(psi), K, nb, out);
+          else
+               hipLaunchKernelGGL((k_block_sums_seq<float>), dim3(grid), dim3(kBlock), 0, s,
+                                  static_cast<const float2*>(psi), K, nb, out);
+     }
+     return hipGetLastError();
+}
+
+}  // namespace hq

@@ -1,0 +1,353 @@
+// pybind11 module `_cppsim_amd`: the Python face of the engine, method for
+// method the reference's `_cppsim_mpi.SimulatorMPI` (_cppsim_mpi.cpp:61-83).
+// It calls only the C ABI (include/hiqsim_amd.h); errors become RuntimeError
+// like the reference's std::runtime_error.  Device work runs with the GIL
+// released.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hiqsim_amd.h"
+
+namespace py = pybind11;
+
+namespace {
+
+using cvec = std::vector<std::complex<double>>;
+
+void ok(int rc)
+{
+     if (rc != 0) throw std::runtime_error(hiqsim_last_error());
+}
+
+std::vector<uint8_t> to_bits(const std::vector<bool>& b)
+{
+     std::vector<uint8_t> r(b.size());
+     for (size_t i = 0; i < b.size(); ++i) r[i] = b[i] ? 1 : 0;
+     return r;
+}
+
+// ProjectQ QubitOperator terms: [([(index, 'X'|'Y'|'Z'), ...], coeff), ...].
+void parse_terms(const py::list& terms, std::vector<uint64_t>& X, std::vector<uint64_t>& Z,
+                 std::vector<double>& C)
+{
+     for (auto item: terms) {
+          auto tc = item.cast<py::sequence>();
+          if (py::len(tc) != 2) throw std::runtime_error("terms must be (term, coeff) pairs");
+          uint64_t x = 0, z = 0;
+          for (auto op: tc[0].cast<py::sequence>()) {
+               auto pr = op.cast<py::sequence>();
+               const int64_t idx = pr[0].cast<int64_t>();
+               const std::string p = pr[1].cast<std::string>();
+               if (idx < 0 || idx >= 64) throw std::runtime_error("Pauli index out of range");
+               const uint64_t bit = uint64_t(1) << idx;
+               if ((x | z) & bit) throw std::runtime_error("repeated qubit in a Pauli term");
+               if (p == "X") x |= bit;
+               else if (p == "Y") {
+                    x |= bit;
+                    z |= bit;
+               }
+               else if (p == "Z") z |= bit;
+               else throw std::runtime_error("unknown Pauli operator '" + p + "'");
+          }
+          const std::complex<double> c = tc[1].cast<std::complex<double>>();
+          X.push_back(x);
+          Z.push_back(z);
+          C.push_back(c.real());
+          C.push_back(c.imag());
+     }
+}
+
+class PySim {
+public:
+     PySim(uint64_t seed, int64_t max_local, int64_t max_cluster, const std::string& precision,
+           int device)
+     {
+          int p;
+          if (precision == "c128" || precision == "complex128") p = HIQSIM_C128;
+          else if (precision == "c64" || precision == "complex64") p = HIQSIM_C64;
+          else throw std::runtime_error("precision must be 'c128' or 'c64'");
+          ok(hiqsim_create(seed, max_local, max_cluster, p, device, &h_));
+     }
+     ~PySim()
+     {
+          if (h_) hiqsim_destroy(h_);
+     }
+
+     std::vector<int64_t> qubit_ids() const
+     {
+          size_t n = 0;
+          ok(hiqsim_num_qubits(h_, &n));
+          std::vector<int64_t> ids(n);
+          ok(hiqsim_get_qubits_ids(h_, ids.data(), n, &n));
+          return ids;
+     }
+     std::vector<int64_t> global_ids() const { return {}; }
+     void set_qubits_perm(const std::vector<int64_t>& ids)
+     {
+          ok(hiqsim_set_qubits_perm(h_, ids.data(), ids.size()));
+     }
+     void swap_qubits(const std::vector<int64_t>& pairs)
+     {
+          ok(hiqsim_swap_qubits(h_, pairs.data(), pairs.size()));
+     }
+     void permute_qubits(const std::vector<int64_t>& ids)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_permute_qubits(h_, ids.data(), ids.size()));
+     }
+     void allocate_qureg(const std::vector<int64_t>& ids, std::complex<double> init)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_allocate_qureg(h_, ids.data(), ids.size(), init.real(), init.imag()));
+     }
+     void allocate_qubit(int64_t id)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_allocate_qubit(h_, id));
+     }
+     void deallocate_qubit(int64_t id)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_deallocate_qubit(h_, id));
+     }
+     std::vector<bool> measure_qubits(const std::vector<int64_t>& ids)
+     {
+          std::vector<uint8_t> out(ids.size());
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_measure_qubits(h_, ids.data(), ids.size(), out.data()));
+          }
+          return std::vector<bool>(out.begin(), out.end());
+     }
+     void apply_controlled_gate(
+         py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> m,
+         const std::vector<int64_t>& ids, const std::vector<int64_t>& ctrls)
+     {
+          if (m.ndim() != 2 || m.shape(0) != m.shape(1))
+               throw std::runtime_error("apply_controlled_gate(): matrix must be square");
+          const auto* d = reinterpret_cast<const double*>(m.data());
+          const size_t dim = static_cast<size_t>(m.shape(0));
+          ok(hiqsim_apply_controlled_gate(h_, d, dim, ids.data(), ids.size(), ctrls.data(),
+                                          ctrls.size()));
+     }
+     void emulate_math(const py::object&, const py::object&, const py::object&)
+     {
+          // SimulatorMPI.hpp:217-225.
+          throw std::runtime_error("SimulatorMPI::emulate_math() is not supported");
+     }
+     std::complex<double> get_amplitude(const std::vector<bool>& bits,
+                                        const std::vector<int64_t>& ids)
+     {
+          auto b = to_bits(bits);
+          double out[2];
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_get_amplitude(h_, b.data(), b.size(), ids.data(), ids.size(), out));
+          }
+          return {out[0], out[1]};
+     }
+     double get_probability(const std::vector<bool>& bits, const std::vector<int64_t>& ids)
+     {
+          auto b = to_bits(bits);
+          double out = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_get_probability(h_, b.data(), b.size(), ids.data(), ids.size(), &out));
+          }
+          return out;
+     }
+     void run()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_run(h_));
+     }
+     double entropy()
+     {
+          double e = 0;
+          py::gil_scoped_release r;
+          ok(hiqsim_entropy(h_, &e));
+          return e;
+     }
+     py::array_t<std::complex<double>> get_state()
+     {
+          uint64_t n = 0;
+          ok(hiqsim_state_size(h_, &n));
+          py::array_t<std::complex<double>> a(static_cast<py::ssize_t>(n));
+          auto* p = reinterpret_cast<double*>(a.mutable_data());
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_get_state(h_, p, n));
+          }
+          return a;
+     }
+     // SimulatorMPI::cheat_local (SimulatorMPI.cpp:543-559): id -> bit
+     // position and a copy of the state.
+     py::tuple cheat_local()
+     {
+          std::map<int64_t, int64_t> id2pos;
+          auto ids = qubit_ids();
+          for (size_t p = 0; p < ids.size(); ++p) id2pos[ids[p]] = static_cast<int64_t>(p);
+          return py::make_tuple(id2pos, get_state());
+     }
+     void set_state(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v)
+     {
+          const auto* p = reinterpret_cast<const double*>(v.data());
+          const uint64_t n = static_cast<uint64_t>(v.size());
+          py::gil_scoped_release r;
+          ok(hiqsim_set_state(h_, p, n));
+     }
+     // set_wavefunction(wf, ids) (_simulator_mpi.py:281-306): the simulator
+     // adopts the ordering of `ids` for the given amplitudes.
+     void set_wavefunction(
+         py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> wf,
+         const std::vector<int64_t>& ids)
+     {
+          auto cur = qubit_ids();
+          if (ids.size() != cur.size())
+               throw std::runtime_error(
+                   "set_wavefunction(): the qureg must contain all allocated qubits");
+          if (static_cast<uint64_t>(wf.size()) != (uint64_t(1) << ids.size()))
+               throw std::runtime_error("set_wavefunction(): wrong number of amplitudes");
+          std::vector<int64_t> a(ids), b(cur);
+          std::sort(a.begin(), a.end());
+          std::sort(b.begin(), b.end());
+          if (a != b)
+               throw std::runtime_error(
+                   "set_wavefunction(): the qureg must be a permutation of the allocated qubits");
+          ok(hiqsim_set_qubits_perm(h_, ids.data(), ids.size()));
+          set_state(wf);
+     }
+     void collapse_wavefunction(const std::vector<int64_t>& ids, const std::vector<bool>& values)
+     {
+          auto v = to_bits(values);
+          py::gil_scoped_release r;
+          ok(hiqsim_collapse_wavefunction(h_, ids.data(), ids.size(), v.data(), v.size()));
+     }
+     double get_expectation_value(const py::list& terms, const std::vector<int64_t>& ids)
+     {
+          std::vector<uint64_t> X, Z;
+          std::vector<double> C;
+          parse_terms(terms, X, Z, C);
+          double out[2] = {0, 0};
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_expectation_pauli(h_, X.data(), Z.data(), C.data(), X.size(), ids.data(),
+                                           ids.size(), out));
+          }
+          return out[0];
+     }
+     std::complex<double> expectation_complex(const py::list& terms,
+                                              const std::vector<int64_t>& ids)
+     {
+          std::vector<uint64_t> X, Z;
+          std::vector<double> C;
+          parse_terms(terms, X, Z, C);
+          double out[2] = {0, 0};
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_expectation_pauli(h_, X.data(), Z.data(), C.data(), X.size(), ids.data(),
+                                           ids.size(), out));
+          }
+          return {out[0], out[1]};
+     }
+     void apply_qubit_operator(const py::list& terms, const std::vector<int64_t>& ids)
+     {
+          std::vector<uint64_t> X, Z;
+          std::vector<double> C;
+          parse_terms(terms, X, Z, C);
+          py::gil_scoped_release r;
+          ok(hiqsim_apply_pauli_sum(h_, X.data(), Z.data(), C.data(), X.size(), ids.data(),
+                                    ids.size()));
+     }
+     void apply_matrix(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> m,
+                       const std::vector<uint32_t>& pos, uint64_t ctrlmask, bool diag)
+     {
+          if (m.ndim() != 2 || m.shape(0) != (py::ssize_t(1) << pos.size()))
+               throw std::runtime_error("apply_matrix(): matrix does not match positions");
+          const auto* d = reinterpret_cast<const double*>(m.data());
+          py::gil_scoped_release r;
+          ok(hiqsim_apply_matrix(h_, d, static_cast<int>(pos.size()), pos.data(), ctrlmask,
+                                 diag ? 1 : 0));
+     }
+     void synchronize()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_synchronize(h_));
+     }
+     void set_timing(bool on) { ok(hiqsim_set_timing(h_, on ? 1 : 0)); }
+     py::tuple kernel_time(int cls)
+     {
+          uint64_t c = 0;
+          double ms = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_kernel_time(h_, cls, &c, &ms));
+          }
+          return py::make_tuple(c, ms);
+     }
+     uint64_t state_size() const
+     {
+          uint64_t n = 0;
+          ok(hiqsim_state_size(h_, &n));
+          return n;
+     }
+
+private:
+     hiqsim_t* h_ = nullptr;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_cppsim_amd, m)
+{
+     m.doc() = "MI355X state-vector engine (drop-in for HiQsimulator's _cppsim_mpi)";
+     m.def("device_count", [] {
+          int n = 0;
+          ok(hiqsim_device_count(&n));
+          return n;
+     });
+     py::class_<PySim>(m, "SimulatorMPI")
+         .def(py::init<uint64_t, int64_t, int64_t, const std::string&, int>(), py::arg("seed"),
+              py::arg("max_local"), py::arg("max_cluster_size"), py::arg("precision") = "c128",
+              py::arg("device") = 0)
+         .def("get_qubits_ids", &PySim::qubit_ids)
+         .def("get_local_qubits_ids", &PySim::qubit_ids)
+         .def("get_global_qubits_ids", &PySim::global_ids)
+         .def("set_qubits_perm", &PySim::set_qubits_perm)
+         .def("swap_qubits", &PySim::swap_qubits)
+         .def("permute_qubits", &PySim::permute_qubits)
+         .def("allocate_qureg", &PySim::allocate_qureg)
+         .def("allocate_qubit", &PySim::allocate_qubit)
+         .def("deallocate_qubit", &PySim::deallocate_qubit)
+         .def("measure_qubits", &PySim::measure_qubits)
+         .def("apply_controlled_gate", &PySim::apply_controlled_gate)
+         .def("emulate_math", &PySim::emulate_math)
+         .def("get_amplitude", &PySim::get_amplitude)
+         .def("get_probability", &PySim::get_probability)
+         .def("run", &PySim::run)
+         .def("entropy", &PySim::entropy)
+         .def("cheat_local", &PySim::cheat_local)
+         .def("collapse_wavefunction", &PySim::collapse_wavefunction)
+         .def("get_state", &PySim::get_state)
+         .def("set_state", &PySim::set_state)
+         .def("set_wavefunction", &PySim::set_wavefunction)
+         .def("get_expectation_value", &PySim::get_expectation_value)
+         .def("expectation_complex", &PySim::expectation_complex)
+         .def("apply_qubit_operator", &PySim::apply_qubit_operator)
+         .def("apply_matrix", &PySim::apply_matrix, py::arg("matrix"), py::arg("pos"),
+              py::arg("ctrlmask") = 0, py::arg("diag") = false)
+         .def("synchronize", &PySim::synchronize)
+         .def("set_timing", &PySim::set_timing)
+         .def("kernel_time", &PySim::kernel_time)
+         .def("state_size", &PySim::state_size);
+}

@@ -1,0 +1,137 @@
+/*
+ * hiqsim_amd.h — C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * Drop-in boundary for HiQsimulator's full-amplitude simulator.  The
+ * reference exposes this path to Python through the pybind11 module
+ * `_cppsim_mpi` (_cppsim_mpi.cpp:61-83), class SimulatorMPI
+ * (src/simulator-mpi/SimulatorMPI.hpp:43-308).  Each entry point below names
+ * the binding and the C++ member it replaces.  Reference paths are relative to
+ * the HiQsimulator source root.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero status on error; the
+ *    message (the reference's std::runtime_error text where one exists) is
+ *    available from hiqsim_last_error() on the calling thread.
+ *  - Matrices are dense, row-major, interleaved complex128:
+ *    m[2*(i*dim+j)] = Re m[i][j], m[2*(i*dim+j)+1] = Im m[i][j]; matrix bit l
+ *    acts on ids[l] (ids[0] is the least significant bit), as in
+ *    SimulatorMPI::ApplyGate.
+ *  - Bit strings / outcomes are one byte per qubit (0 or 1).
+ *  - State vectors crossing the boundary are interleaved complex128 in host
+ *    memory; index bit p <-> the qubit hiqsim_get_qubits_ids() lists at p.
+ *  - A handle owns one device state and one HIP stream; it is not
+ *    thread-safe (the reference object is not either).
+ */
+#ifndef HIQSIM_AMD_H
+#define HIQSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hiqsim_engine hiqsim_t;
+
+enum hiqsim_precision { HIQSIM_C128 = 0, HIQSIM_C64 = 1 };
+
+/* ABI version of this header. */
+int hiqsim_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* hiqsim_last_error(void);
+/* Number of visible HIP devices. */
+int hiqsim_device_count(int* out);
+
+/* SimulatorMPI(seed, max_local, max_cluster_size)
+ * (_cppsim_mpi.cpp:64, SimulatorMPI.cpp:66-104).  precision: HIQSIM_C128 or
+ * HIQSIM_C64 (amplitudes only; fusion, norms and probabilities stay FP64). */
+int hiqsim_create(uint64_t seed, int64_t max_local, int64_t max_cluster_size, int precision,
+                  int device, hiqsim_t** out);
+int hiqsim_destroy(hiqsim_t* h);
+
+/* allocate_qubit (_cppsim_mpi.cpp:72, SimulatorMPI.cpp:185-216). */
+int hiqsim_allocate_qubit(hiqsim_t* h, int64_t id);
+/* allocate_qureg (_cppsim_mpi.cpp:71, SimulatorMPI.cpp:218-251). */
+int hiqsim_allocate_qureg(hiqsim_t* h, const int64_t* ids, size_t n, double init_re,
+                          double init_im);
+/* deallocate_qubit (_cppsim_mpi.cpp:73, SimulatorMPI.cpp:369-426). */
+int hiqsim_deallocate_qubit(hiqsim_t* h, int64_t id);
+
+/* apply_controlled_gate (_cppsim_mpi.cpp:75, SimulatorMPI.cpp:710-815):
+ * deferred until hiqsim_run; dim must equal 2^nids. */
+int hiqsim_apply_controlled_gate(hiqsim_t* h, const double* m, size_t dim, const int64_t* ids,
+                                 size_t nids, const int64_t* ctrls, size_t nctrls);
+/* run (_cppsim_mpi.cpp:79, SimulatorMPI.cpp:441-541): fuse and apply. */
+int hiqsim_run(hiqsim_t* h);
+
+/* measure_qubits (_cppsim_mpi.cpp:74, SimulatorMPI.cpp:897-1008). */
+int hiqsim_measure_qubits(hiqsim_t* h, const int64_t* ids, size_t n, uint8_t* out);
+/* get_probability (_cppsim_mpi.cpp:78, SimulatorMPI.cpp:561-600). */
+int hiqsim_get_probability(hiqsim_t* h, const uint8_t* bits, size_t nbits, const int64_t* ids,
+                           size_t nids, double* out);
+/* get_amplitude (_cppsim_mpi.cpp:77, SimulatorMPI.cpp:602-650); out = {re, im}. */
+int hiqsim_get_amplitude(hiqsim_t* h, const uint8_t* bits, size_t nbits, const int64_t* ids,
+                         size_t nids, double* out);
+/* collapse_wavefunction (_cppsim_mpi.cpp:82, SimulatorMPI.cpp:1010-1058). */
+int hiqsim_collapse_wavefunction(hiqsim_t* h, const int64_t* ids, size_t nids,
+                                 const uint8_t* values, size_t nvalues);
+/* entropy (_cppsim_mpi.cpp:80, SimulatorMPI.cpp:681-694). */
+int hiqsim_entropy(hiqsim_t* h, double* out);
+
+/* Number of allocated qubits (SimulatorMPI.hpp:275-297). */
+int hiqsim_num_qubits(hiqsim_t* h, size_t* out);
+/* get_qubits_ids / get_local_qubits_ids (_cppsim_mpi.cpp:65-66,
+ * SimulatorMPI.cpp:652-659): id of the qubit at each bit position.  One
+ * device has no global qubits (get_global_qubits_ids is empty). */
+int hiqsim_get_qubits_ids(hiqsim_t* h, int64_t* out, size_t cap, size_t* n);
+/* set_qubits_perm (_cppsim_mpi.cpp:69, SimulatorMPI.cpp:661-667): relabel only. */
+int hiqsim_set_qubits_perm(hiqsim_t* h, const int64_t* ids, size_t n);
+/* swap_qubits (_cppsim_mpi.cpp:70, SimulatorMPI.cpp:1060-1138): (global, local)
+ * pairs; on one device there are no global qubits, so any pair is an error. */
+int hiqsim_swap_qubits(hiqsim_t* h, const int64_t* pairs, size_t n);
+/* In-HBM qubit permutation replacing the data movement of the reference's
+ * qswap (SwapperMT::doSwap, SwapperMT.cpp:100-126; swapping.hpp:33-67):
+ * afterwards ids[p] is the qubit at bit p.  Bit-exact. */
+int hiqsim_permute_qubits(hiqsim_t* h, const int64_t* ids, size_t n);
+
+/* Number of amplitudes (2^num_qubits). */
+int hiqsim_state_size(hiqsim_t* h, uint64_t* out);
+/* cheat_local (_cppsim_mpi.cpp:81, SimulatorMPI.cpp:543-559): copy the state
+ * out as interleaved complex128 (n_amps must equal hiqsim_state_size). */
+int hiqsim_get_state(hiqsim_t* h, double* out, uint64_t n_amps);
+/* set_wavefunction (_simulator_mpi.py:281-306; declared but unbound in the
+ * reference): replace the amplitudes, qubit order unchanged. */
+int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps);
+
+/* get_expectation_value (_simulator_mpi.py:148-183; unbound in the
+ * reference): sum_t coeff[t] <psi|P_t|psi>.  Bit i of xmask[t]/zmask[t] refers
+ * to ids[i]; X = x only, Z = z only, Y = both.  coeff and out are {re, im}. */
+int hiqsim_expectation_pauli(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
+                             const double* coeff, size_t nterms, const int64_t* ids, size_t nids,
+                             double* out);
+/* apply_qubit_operator (_simulator_mpi.py:185-223; unbound in the reference):
+ * psi <- sum_t coeff[t] P_t psi (not renormalised). */
+int hiqsim_apply_pauli_sum(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
+                           const double* coeff, size_t nterms, const int64_t* ids, size_t nids);
+
+/* One immediate fused pass at explicit bit positions, bypassing fusion: the
+ * reference's kernelK / kernel_core(_diag) call in Run()
+ * (SimulatorMPI.cpp:470-514).  k = number of positions (1..5), m is 2^k x 2^k;
+ * diag != 0 selects the diagonal kernel when m is diagonal. */
+int hiqsim_apply_matrix(hiqsim_t* h, const double* m, int k, const uint32_t* pos,
+                        uint64_t ctrlmask, int diag);
+
+/* Wait for all queued device work of this handle. */
+int hiqsim_synchronize(hiqsim_t* h);
+/* HIP-event timing of every kernel launch on the handle's stream (resets the
+ * counters).  Classes: 0-4 dense k=1..5, 5 diagonal, 6 scale, 7 permute,
+ * 8 reductions, 9 normalize, 10 bit removal, 11 fill/copy, 12 Pauli. */
+int hiqsim_set_timing(hiqsim_t* h, int on);
+int hiqsim_kernel_time(hiqsim_t* h, int cls, uint64_t* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIQSIM_AMD_H */

@@ -1,0 +1,221 @@
+"""Engine-level parity on the GPU: the reference test-suite's known answers,
+and differential runs against the oracle on identical API call sequences.
+
+complex128 states: max-abs <= 1e-12 against the FP64 oracle.
+Measurement: identical outcomes for the same seed (same mt19937 u stream and
+block-CDF search as the reference, SimulatorMPI.cpp:897-1008).
+Permutation: bit-exact.  complex64: fidelity >= 1 - 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import scenarios
+from hiqsimulator_amd.circuits import (H, X, apply_circuit_inorder, haar_unitary,
+                                       qft_like_circuit, random_layers_circuit,
+                                       random_permutation)
+
+pytestmark = pytest.mark.gpu
+
+
+def make_engine(seed, max_local, max_cluster):
+    import hiqsimulator_amd
+    return hiqsimulator_amd.SimulatorMPI(seed, max_local, max_cluster)
+
+
+@pytest.mark.parametrize("scenario", scenarios.ALL_KNOWN_ANSWERS, ids=lambda f: f.__name__)
+def test_known_answers(engine_mod, scenario):
+    print("scenario", scenario.__name__, flush=True)
+    scenario(make_engine)
+
+
+def test_ghz_measurement_matches_oracle(engine_mod, oracle_mod):
+    seen = set()
+    for seed in range(10):
+        a = scenarios.ghz_measure(make_engine, seed)
+        b = scenarios.ghz_measure(lambda s, ml, mc: oracle_mod.OracleSim(s, ml, mc), seed)
+        assert a == b
+        seen.add(sum(a))
+    assert seen == {0, 5}
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5])
+def test_config0_random_circuit(engine_mod, oracle_mod, k):
+    """16 qubits, 20 layers, |0..0>, one seed: full state vs oracle."""
+    n, layers, seed = 16, 20, 2024
+    gates = random_layers_circuit(n, layers, seed)
+    g = make_engine(seed, n, k)
+    o = oracle_mod.OracleSim(seed, n, k)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        apply_circuit_inorder(s, gates, k)
+    a, b = g.get_state(), o.get_state()
+    assert np.abs(a - b).max() <= 1e-12
+    assert abs(np.linalg.norm(a) - 1.0) < 1e-12
+
+
+def test_fusion_with_folded_controls(engine_mod, oracle_mod):
+    """Gates fused into one pass, with controls folded into the matrix, and
+    a wide controlled gate taking the control-mask path."""
+    rng = np.random.default_rng(5)
+    n = 9
+    g = make_engine(1, n, 4)
+    o = oracle_mod.OracleSim(1, n, 4)
+    seq = [(haar_unitary(2, rng), [3], [7]), (haar_unitary(4, rng), [1, 3], []),
+           (H, [7], []), (haar_unitary(2, rng), [1], [3, 7]), ("run",),
+           (X, [8], [0, 2, 4, 6]), ("run",), (haar_unitary(8, rng), [5, 0, 2], []),
+           (haar_unitary(2, rng), [4], [5]), ("run",), (np.diag([1, 1j]), [6], [2, 3]),
+           (np.diag([1j, -1]), [6], []), ("run",)]
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        for q in range(n):
+            s.apply_controlled_gate(H, [q], [])
+            s.run()
+        for item in seq:
+            if item[0] == "run":
+                s.run()
+            else:
+                s.apply_controlled_gate(*item)
+    assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+
+
+def test_measure_collapse_dealloc_sequence(engine_mod, oracle_mod):
+    n = 12
+    gates = random_layers_circuit(n, 8, seed=77)
+    g = make_engine(4242, n, 4)
+    o = oracle_mod.OracleSim(4242, n, 4)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        apply_circuit_inorder(s, gates, 4)
+    for q in [3, 0, 11, 7]:
+        assert g.measure_qubits([q]) == o.measure_qubits([q])
+        assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+    assert g.measure_qubits([1, 2, 5]) == o.measure_qubits([1, 2, 5])
+    for q in [11, 3, 0, 7, 1]:
+        g.deallocate_qubit(q)
+        o.deallocate_qubit(q)
+        assert g.get_qubits_ids() == o.get_qubits_ids()
+        assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+    assert g.entropy() == pytest.approx(o.entropy(), abs=1e-12)
+    ids = g.get_qubits_ids()
+    for bits in ([0] * len(ids), [1, 0, 1, 0, 1, 0, 1][:len(ids)]):
+        assert g.get_probability(bits, ids) == pytest.approx(o.get_probability(bits, ids),
+                                                            abs=1e-13)
+        assert g.get_amplitude(bits, ids) == pytest.approx(o.get_amplitude(bits, ids), abs=1e-13)
+
+
+def test_measurement_stream_many_seeds(engine_mod, oracle_mod):
+    """Same seed, same u stream, same block-CDF search: identical outcomes on
+    a 17-qubit state (more than 2^15 amplitudes, so blocks hold >1 entry)."""
+    n = 17
+    gates = random_layers_circuit(n, 4, seed=9)
+    for seed in (0, 1, 99, 2 ** 32 + 5):
+        g = make_engine(seed, n, 4)
+        o = oracle_mod.OracleSim(seed, n, 4)
+        for s in (g, o):
+            s.allocate_qureg(list(range(n)), 0)
+            apply_circuit_inorder(s, gates, 4)
+        for q in range(0, n, 3):
+            assert g.measure_qubits([q, (q + 1) % n]) == o.measure_qubits([q, (q + 1) % n])
+        assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+
+
+def test_expectation_and_operator_vs_oracle(engine_mod, oracle_mod):
+    rng = np.random.default_rng(12)
+    n = 10
+    gates = random_layers_circuit(n, 5, seed=12)
+    g = make_engine(1, n, 4)
+    o = oracle_mod.OracleSim(1, n, 4)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        apply_circuit_inorder(s, gates, 4)
+    order = [int(x) for x in rng.permutation(n)]
+    for t in range(8):
+        w = int(rng.integers(1, 5))
+        qs = rng.choice(n, size=w, replace=False)
+        term = [(int(q), "XYZ"[int(rng.integers(0, 3))]) for q in qs]
+        ops = [(term, complex(rng.normal(), rng.normal())), ([], 0.25)]
+        assert g.get_expectation_value(ops, order) == pytest.approx(
+            o.get_expectation_value(ops, order), abs=1e-12)
+    ops = [([(0, "X"), (3, "Y")], 0.5), ([(2, "Z")], -0.25j), ([], 0.1)]
+    g.apply_qubit_operator(ops, order)
+    o.apply_qubit_operator(ops, order)
+    assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+
+
+def test_permutations_keep_results(engine_mod, oracle_mod):
+    """Random qubit permutations between circuit halves: state (read in the
+    permuted order) and probabilities agree with the oracle, which also
+    permutes, and the GPU data movement is bit-exact."""
+    n = 14
+    rng = np.random.default_rng(3)
+    gates = random_layers_circuit(n, 10, seed=3)
+    g = make_engine(1, n, 4)
+    o = oracle_mod.OracleSim(1, n, 4)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+    half = len(gates) // 2
+    for part in (gates[:half], gates[half:]):
+        for s in (g, o):
+            apply_circuit_inorder(s, part, 4)
+        perm = random_permutation(g.get_qubits_ids(), rng)
+        before = g.get_state()
+        g.permute_qubits(perm)
+        o.permute_qubits(perm)
+        after = g.get_state()
+        assert np.array_equal(np.sort_complex(before), np.sort_complex(after))
+        assert g.get_qubits_ids() == o.get_qubits_ids() == perm
+        assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+
+
+def test_complex64_fidelity(engine_mod, oracle_mod):
+    n = 16
+    gates = random_layers_circuit(n, 20, seed=5)
+    g = engine_mod.SimulatorMPI(5, n, 4, "c64")
+    o = oracle_mod.OracleSim(5, n, 4)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        apply_circuit_inorder(s, gates, 4)
+    a, b = g.get_state(), o.get_state()
+    fid = abs(np.vdot(b, a)) ** 2
+    print("c64 16q max-abs %.3e fidelity 1-%.3e" % (np.abs(a - b).max(), 1 - fid))
+    assert fid >= 1 - 1e-6
+
+
+def test_qft_equals_dft(engine_mod):
+    """QFT-like circuit == normalised DFT up to a bit reversal of the output
+    index (closed form), on a random normalised state."""
+    n = 12
+    rng = np.random.default_rng(8)
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    st /= np.linalg.norm(st)
+    g = make_engine(1, n, 4)
+    g.allocate_qureg(list(range(n)), 0)
+    g.set_state(st)
+    apply_circuit_inorder(g, qft_like_circuit(n), 4)
+    got = g.get_state()
+    dft = np.fft.ifft(st, norm="ortho")
+    rev = np.array([int(format(i, "0%db" % n)[::-1], 2) for i in range(1 << n)])
+    assert np.abs(got[rev] - dft).max() <= 1e-12
+
+
+def test_large_state_roundtrip_26q(engine_mod):
+    """Size-independent properties at 26 qubits: a random circuit followed by
+    its inverse returns |0..0>; a permutation and its inverse are bit-exact."""
+    n = 26
+    gates = random_layers_circuit(n, 4, seed=26)
+    inv = [(m.conj().T, t, c) for (m, t, c) in reversed(gates)]
+    g = make_engine(1, n, 5)
+    g.allocate_qureg(list(range(n)), 0)
+    apply_circuit_inorder(g, gates, 5)
+    assert g.get_probability([0] * n, list(range(n))) < 0.5
+    rng = np.random.default_rng(1)
+    before = g.get_amplitude([1, 0] * 13, list(range(n)))
+    perm = random_permutation(list(range(n)), rng)
+    g.permute_qubits(perm)
+    g.permute_qubits(list(range(n)))
+    assert g.get_amplitude([1, 0] * 13, list(range(n))) == before
+    apply_circuit_inorder(g, inv, 5)
+    assert abs(g.get_amplitude([0] * n, list(range(n))) - 1.0) < 1e-10
+    assert abs(g.get_probability([0] * n, list(range(n))) - 1.0) < 1e-10

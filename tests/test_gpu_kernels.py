@@ -1,0 +1,114 @@
+"""Kernel-level parity: each HIP pass vs the oracle on the same seeded input.
+
+complex128: max-abs <= 1e-12 (scaled by the state's magnitude).
+complex64:  max-abs relative to the FP64 oracle <= 2e-6 per pass.
+Permutation: bit-exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_state(rng, n):
+    st = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return st / np.linalg.norm(st)
+
+
+def rand_case(rng, n, k, nctrl, diag):
+    pos = [int(p) for p in rng.permutation(n)[:k]]
+    free = [p for p in range(n) if p not in pos]
+    cm = 0
+    for p in rng.permutation(free)[:nctrl]:
+        cm |= 1 << int(p)
+    dim = 1 << k
+    m = rng.normal(size=(dim, dim)) + 1j * rng.normal(size=(dim, dim))
+    if diag:
+        m = np.diag(np.diag(m))
+    return pos, cm, m
+
+
+def engine_with_state(eng, st, precision="c128"):
+    n = int(np.log2(st.size))
+    sim = eng.SimulatorMPI(1, 40, 5, precision)
+    sim.allocate_qureg(list(range(n)), 0)
+    sim.set_state(st)
+    return sim
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 7, 9, 12, 14, 17])
+def test_dense_and_diag_c128(engine_mod, oracle_mod, n):
+    rng = np.random.default_rng(100 + n)
+    for k in range(1, min(5, n) + 1):
+        for nctrl in sorted({0, min(1, n - k), min(2, n - k)}):
+            for diag in (False, True):
+                print("case n=%d k=%d nctrl=%d diag=%d" % (n, k, nctrl, diag), flush=True)
+                pos, cm, m = rand_case(rng, n, k, nctrl, diag)
+                st = rand_state(rng, n)
+                sim = engine_with_state(engine_mod, st)
+                sim.apply_matrix(m, pos, cm, diag)
+                got = sim.get_state()
+                ref = st.copy()
+                (oracle_mod.diag_pass if diag else oracle_mod.dense_pass)(ref, m, pos, cm)
+                err = np.abs(got - ref).max()
+                assert err <= 1e-12 * max(1.0, np.abs(ref).max()), (n, k, pos, cm, diag, err)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_dense_positions_sweep_20q(engine_mod, oracle_mod, k):
+    """All target placements that matter for coalescing: low, mixed, high."""
+    n = 20
+    rng = np.random.default_rng(7 * k)
+    placements = [list(range(k)), list(range(n - k, n)), [int(p) for p in rng.permutation(n)[:k]],
+                  [0] + list(range(n - k + 1, n))]
+    st = rand_state(rng, n)
+    for pos in placements:
+        print("case k=%d pos=%s" % (k, pos), flush=True)
+        m = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        sim = engine_with_state(engine_mod, st)
+        sim.apply_matrix(m, pos, 0, False)
+        got = sim.get_state()
+        ref = st.copy()
+        oracle_mod.dense_pass(ref, m, pos, 0)
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), pos
+
+
+@pytest.mark.parametrize("n", [4, 11, 16])
+def test_dense_and_diag_c64(engine_mod, oracle_mod, n):
+    from hiqsimulator_amd.circuits import haar_unitary
+    rng = np.random.default_rng(200 + n)
+    for k in range(1, min(5, n) + 1):
+        for diag in (False, True):
+            pos, cm, _ = rand_case(rng, n, k, 1 if n > k else 0, diag)
+            m = haar_unitary(1 << k, rng)
+            if diag:
+                m = np.diag(np.exp(1j * rng.uniform(0, 6.28, size=1 << k)))
+            st = rand_state(rng, n)
+            sim = engine_with_state(engine_mod, st, "c64")
+            sim.apply_matrix(m, pos, cm, diag)
+            got = sim.get_state()
+            ref = st.copy()
+            (oracle_mod.diag_pass if diag else oracle_mod.dense_pass)(ref, m, pos, cm)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err <= 2e-6, (n, k, diag, err)
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("n", [1, 3, 8, 15, 21])
+def test_permutation_bit_exact(engine_mod, n, precision):
+    rng = np.random.default_rng(300 + n)
+    st = rand_state(rng, n)
+    if precision == "c64":
+        st = st.astype(np.complex64).astype(np.complex128)
+    sim = engine_with_state(engine_mod, st, precision)
+    new = [int(x) for x in rng.permutation(n)]
+    sim.permute_qubits(new)
+    got = sim.get_state()
+    idx = np.zeros(1 << n, dtype=np.int64)
+    i = np.arange(1 << n, dtype=np.int64)
+    for p, q in enumerate(new):
+        idx |= ((i >> q) & 1) << p
+    expect = np.empty_like(st)
+    expect[idx] = st
+    assert np.array_equal(got, expect)
+    assert sim.get_qubits_ids() == new
