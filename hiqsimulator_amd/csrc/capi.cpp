@@ -6,6 +6,7 @@
 
 #include "../../include/hiqsim_amd.h"
 #include "engine.hpp"
+#include "sched.hpp"
 
 struct hiqsim_engine {
      hq::Engine eng;
@@ -72,6 +73,23 @@ void pauli_args(const uint64_t* x, const uint64_t* z, const double* c, size_t nt
      Z.assign(z, z + nt);
      C.resize(nt);
      for (size_t t = 0; t < nt; ++t) C[t] = hq::cplx(c[2 * t], c[2 * t + 1]);
+}
+
+hq::GateList gate_list(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
+                       const size_t* ctrl_off, const uint8_t* diag, size_t ngates)
+{
+     need(gate_off, "gate_off");
+     need(ctrl_off, "ctrl_off");
+     if (ngates) need(diag, "diag");
+     hq::GateList g;
+     for (size_t i = 0; i < ngates; ++i) {
+          if (gate_off[i + 1] < gate_off[i] || ctrl_off[i + 1] < ctrl_off[i])
+               throw std::runtime_error("scheduler: offsets must be non-decreasing");
+          g.gate.push_back(ivec(gate_ids + gate_off[i], gate_off[i + 1] - gate_off[i]));
+          g.ctrl.push_back(ivec(ctrl_ids + ctrl_off[i], ctrl_off[i + 1] - ctrl_off[i]));
+          g.diag.push_back(diag[i] != 0);
+     }
+     return g;
 }
 
 }  // namespace
@@ -342,6 +360,37 @@ int hiqsim_kernel_time(hiqsim_t* h, int cls, uint64_t* count, double* total_ms)
           need(count, "count");
           need(total_ms, "total_ms");
           h->eng.kernel_time(cls, *count, *total_ms);
+     });
+}
+
+int hiqsim_schedule_cluster(const int64_t* gate_ids, const size_t* gate_off,
+                            const int64_t* ctrl_ids, const size_t* ctrl_off, const uint8_t* diag,
+                            size_t ngates, const int64_t* locals, size_t nlocals,
+                            const int64_t* globals, size_t nglobals, int cluster_size,
+                            int32_t* out, size_t* nout)
+{
+     return guard([&] {
+          need(nout, "nout");
+          auto g = gate_list(gate_ids, gate_off, ctrl_ids, ctrl_off, diag, ngates);
+          auto r = hq::schedule_cluster(g, ivec(locals, nlocals), ivec(globals, nglobals),
+                                        cluster_size);
+          if (!r.empty()) need(out, "out");
+          for (size_t i = 0; i < r.size(); ++i) out[i] = r[i];
+          *nout = r.size();
+     });
+}
+
+int hiqsim_schedule_swap(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, const uint8_t* diag, size_t ngates,
+                         int64_t num_splits, int num_locals, int fuse, int64_t* out, size_t* nout)
+{
+     return guard([&] {
+          need(nout, "nout");
+          auto g = gate_list(gate_ids, gate_off, ctrl_ids, ctrl_off, diag, ngates);
+          auto r = hq::schedule_swap(g, num_splits, num_locals, fuse != 0);
+          if (!r.empty()) need(out, "out");
+          for (size_t i = 0; i < r.size(); ++i) out[i] = r[i];
+          *nout = r.size();
      });
 }
 

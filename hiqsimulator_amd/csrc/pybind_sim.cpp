@@ -306,6 +306,81 @@ private:
      hiqsim_t* h_ = nullptr;
 };
 
+struct CsrGates {
+     std::vector<int64_t> gid, cid;
+     std::vector<size_t> goff{0}, coff{0};
+     std::vector<uint8_t> diag;
+     CsrGates(const std::vector<std::vector<int64_t>>& gate,
+              const std::vector<std::vector<int64_t>>& ctrl, const std::vector<bool>& d)
+     {
+          if (gate.size() != ctrl.size() || gate.size() != d.size())
+               throw std::runtime_error("ctor(): gate, gate_ctrl and gate_diag differ in length");
+          for (size_t i = 0; i < gate.size(); ++i) {
+               gid.insert(gid.end(), gate[i].begin(), gate[i].end());
+               cid.insert(cid.end(), ctrl[i].begin(), ctrl[i].end());
+               goff.push_back(gid.size());
+               coff.push_back(cid.size());
+               diag.push_back(d[i] ? 1 : 0);
+          }
+     }
+};
+
+// _sched_cpp.ClusterScheduler (_sched_cpp.cpp:36-39).
+class PyClusterScheduler {
+public:
+     PyClusterScheduler(std::vector<std::vector<int64_t>> gate,
+                        std::vector<std::vector<int64_t>> ctrl, std::vector<bool> diag,
+                        std::vector<int64_t> locals, std::vector<int64_t> globals, int size)
+         : g_(gate, ctrl, diag), locals_(std::move(locals)), globals_(std::move(globals)), size_(size)
+     {}
+     std::vector<int> schedule()
+     {
+          std::vector<int32_t> out(g_.diag.size() + 1);
+          size_t n = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_schedule_cluster(g_.gid.data(), g_.goff.data(), g_.cid.data(),
+                                          g_.coff.data(), g_.diag.data(), g_.diag.size(),
+                                          locals_.data(), locals_.size(), globals_.data(),
+                                          globals_.size(), size_, out.data(), &n));
+          }
+          return std::vector<int>(out.begin(), out.begin() + static_cast<long>(n));
+     }
+
+private:
+     CsrGates g_;
+     std::vector<int64_t> locals_, globals_;
+     int size_;
+};
+
+// _sched_cpp.SwapScheduler (_sched_cpp.cpp:31-35).
+class PySwapScheduler {
+public:
+     PySwapScheduler(std::vector<std::vector<int64_t>> gate, std::vector<std::vector<int64_t>> ctrl,
+                     std::vector<bool> diag, int64_t num_splits, int num_locals, bool fuse)
+         : g_(gate, ctrl, diag), splits_(num_splits), locals_(num_locals), fuse_(fuse)
+     {}
+     std::vector<int64_t> schedule()
+     {
+          std::vector<int64_t> out(64);
+          size_t n = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_schedule_swap(g_.gid.data(), g_.goff.data(), g_.cid.data(), g_.coff.data(),
+                                       g_.diag.data(), g_.diag.size(), splits_, locals_,
+                                       fuse_ ? 1 : 0, out.data(), &n));
+          }
+          out.resize(n);
+          return out;
+     }
+
+private:
+     CsrGates g_;
+     int64_t splits_;
+     int locals_;
+     bool fuse_;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_cppsim_amd, m)
@@ -350,4 +425,12 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("set_timing", &PySim::set_timing)
          .def("kernel_time", &PySim::kernel_time)
          .def("state_size", &PySim::state_size);
+     py::class_<PySwapScheduler>(m, "SwapScheduler")
+         .def(py::init<std::vector<std::vector<int64_t>>, std::vector<std::vector<int64_t>>,
+                       std::vector<bool>, int64_t, int, bool>())
+         .def("ScheduleSwap", &PySwapScheduler::schedule);
+     py::class_<PyClusterScheduler>(m, "ClusterScheduler")
+         .def(py::init<std::vector<std::vector<int64_t>>, std::vector<std::vector<int64_t>>,
+                       std::vector<bool>, std::vector<int64_t>, std::vector<int64_t>, int>())
+         .def("ScheduleCluster", &PyClusterScheduler::schedule);
 }

@@ -1,0 +1,340 @@
+// Gate schedulers; see sched.hpp for the contract.
+#include "sched.hpp"
+
+#include <algorithm>
+#include <map>
+#include <stdexcept>
+#include <string>
+
+namespace hq {
+namespace {
+
+inline int popc(msk_t x) { return __builtin_popcountll(x); }
+inline bool submask(msk_t a, msk_t b) { return (a & ~b) == 0; }
+
+// Dense renumbering of every id that appears (convertors.cpp:42-76).
+struct Positions {
+     std::vector<int64_t> pos_to_id;
+     std::map<int64_t, int> id_to_pos;
+
+     Positions(const GateList& g, const std::vector<int64_t>& a, const std::vector<int64_t>& b)
+     {
+          for (auto& v: g.gate) pos_to_id.insert(pos_to_id.end(), v.begin(), v.end());
+          for (auto& v: g.ctrl) pos_to_id.insert(pos_to_id.end(), v.begin(), v.end());
+          pos_to_id.insert(pos_to_id.end(), a.begin(), a.end());
+          pos_to_id.insert(pos_to_id.end(), b.begin(), b.end());
+          std::sort(pos_to_id.begin(), pos_to_id.end());
+          pos_to_id.erase(std::unique(pos_to_id.begin(), pos_to_id.end()), pos_to_id.end());
+          if (pos_to_id.size() >= 64)
+               throw std::runtime_error(
+                   "CalcPos(): scheduling with 64 or more qubits is not yet supported");
+          for (size_t i = 0; i < pos_to_id.size(); ++i) id_to_pos[pos_to_id[i]] = static_cast<int>(i);
+     }
+     msk_t mask(const std::vector<int64_t>& ids) const
+     {
+          msk_t m = 0;
+          for (auto id: ids) m |= msk_t(1) << id_to_pos.at(id);
+          return m;
+     }
+};
+
+void check_sizes(const GateList& g)
+{
+     if (g.gate.size() != g.ctrl.size() || g.gate.size() != g.diag.size())
+          throw std::runtime_error("scheduler: gate, gate_ctrl and gate_diag differ in length");
+}
+
+// Open-addressing map msk_t -> msk_t (the "used" bits per cluster).  Key 0
+// is a valid cluster, so occupancy is tracked separately.
+class MaskMap {
+public:
+     MaskMap() { rehash(1 << 12); }
+     msk_t& at(msk_t key)
+     {
+          if ((size_ + 1) * 2 > keys_.size()) rehash(keys_.size() * 2);
+          size_t i = slot(key);
+          if (!full_[i]) {
+               full_[i] = 1;
+               keys_[i] = key;
+               vals_[i] = 0;
+               ++size_;
+          }
+          return vals_[i];
+     }
+
+private:
+     size_t slot(msk_t key) const
+     {
+          const size_t mask = keys_.size() - 1;
+          size_t i = static_cast<size_t>((key * 0x9E3779B97F4A7C15ull) >> 20) & mask;
+          while (full_[i] && keys_[i] != key) i = (i + 1) & mask;
+          return i;
+     }
+     void rehash(size_t cap)
+     {
+          std::vector<msk_t> ok = std::move(keys_), ov = std::move(vals_);
+          std::vector<uint8_t> of = std::move(full_);
+          keys_.assign(cap, 0);
+          vals_.assign(cap, 0);
+          full_.assign(cap, 0);
+          size_ = 0;
+          for (size_t i = 0; i < of.size(); ++i)
+               if (of[i]) {
+                    size_t j = slot(ok[i]);
+                    full_[j] = 1;
+                    keys_[j] = ok[i];
+                    vals_[j] = ov[i];
+                    ++size_;
+               }
+     }
+     std::vector<msk_t> keys_, vals_;
+     std::vector<uint8_t> full_;
+     size_t size_ = 0;
+};
+
+class Cluster {
+public:
+     Cluster(const GateList& g, const std::vector<int64_t>& locals,
+             const std::vector<int64_t>& globals, int cluster_size)
+         : size_(cluster_size), diag_(g.diag)
+     {
+          check_sizes(g);
+          Positions p(g, locals, globals);
+          for (size_t i = 0; i < g.gate.size(); ++i) {
+               gate_.push_back(p.mask(g.gate[i]));
+               ctrl_.push_back(p.mask(g.ctrl[i]));
+          }
+          locals_ = p.mask(locals);
+          // nonlocal_after_[i]: some gate at index >= i has no local qubit.
+          nonlocal_after_.assign(gate_.size() + 1, 0);
+          for (size_t i = gate_.size(); i-- > 0;)
+               nonlocal_after_[i] = nonlocal_after_[i + 1] || (((gate_[i] | ctrl_[i]) & locals_) == 0);
+     }
+
+     std::vector<int> run()
+     {
+          if (gate_.empty()) return {};
+          best_ans_ = 0;
+          best_ = 0;
+          for (size_t i = 0; i < gate_.size(); ++i) {
+               const msk_t cur = locals_ & (gate_[i] | ctrl_[i]);
+               if (popc(cur) <= size_) rec(cur, 1);
+          }
+          if (best_ans_ != 0) return take(best_);
+          msk_t bad = 0;
+          for (size_t i = 0; i < gate_.size(); ++i) {
+               if (can_take(i, locals_, bad)) return {static_cast<int>(i)};
+               bad |= gate_[i] | ctrl_[i];
+          }
+          return {};
+     }
+
+private:
+     bool can_take(size_t i, msk_t cur, msk_t bad) const
+     {
+          const msk_t all = gate_[i] | ctrl_[i];
+          if ((all & bad) || !submask(all & locals_, cur)) return false;
+          return diag_[i] ? true : submask(gate_[i], locals_);
+     }
+
+     // cluster_scheduler.h:76-99 (CalcCombination).
+     void evaluate(msk_t cur)
+     {
+          msk_t bad = 0;
+          int ans = 0;
+          for (size_t i = 0; i < gate_.size(); ++i) {
+               if (can_take(i, cur, bad)) ++ans;
+               else bad |= gate_[i] | ctrl_[i];
+               if (submask(cur, bad) && !nonlocal_after_[i + 1]) break;
+          }
+          if (ans > best_ans_ || (ans == best_ans_ && popc(cur) < popc(best_))) {
+               best_ans_ = ans;
+               best_ = cur;
+          }
+     }
+
+     // cluster_scheduler.cpp:80-99 (Rec); the memo records, per cluster,
+     // which "next bit" expansions were done; a cluster is evaluated once.
+     void rec(msk_t cur, msk_t bit)
+     {
+          msk_t& used = memo_.at(cur);
+          if (used == 0) evaluate(cur);
+          if (bit > locals_) {
+               if (used == 0) used = msk_t(1) << 63;  // evaluated, nothing expanded
+               return;
+          }
+          if (used & bit) return;
+          used |= bit;
+          if (submask(bit, locals_) && !(bit & cur) && popc(cur) < size_) rec(cur | bit, bit << 1);
+          rec(cur, bit << 1);
+     }
+
+     std::vector<int> take(msk_t cluster) const
+     {
+          std::vector<int> ans;
+          msk_t bad = 0;
+          for (size_t i = 0; i < gate_.size(); ++i) {
+               if (can_take(i, cluster, bad)) ans.push_back(static_cast<int>(i));
+               else bad |= gate_[i] | ctrl_[i];
+          }
+          return ans;
+     }
+
+     int size_;
+     std::vector<bool> diag_;
+     std::vector<msk_t> gate_, ctrl_;
+     std::vector<uint8_t> nonlocal_after_;
+     msk_t locals_ = 0;
+     int best_ans_ = 0;
+     msk_t best_ = 0;
+     MaskMap memo_;
+};
+
+class Swap {
+public:
+     Swap(const GateList& g, int64_t num_splits, int num_locals, bool fuse)
+         : num_splits_(num_splits), num_locals_(num_locals), diag_(g.diag), weight_(g.gate.size(), 1),
+           pos_(g, {}, {})
+     {
+          check_sizes(g);
+          for (size_t i = 0; i < g.gate.size(); ++i) {
+               gate_.push_back(pos_.mask(g.gate[i]));
+               ctrl_.push_back(pos_.mask(g.ctrl[i]));
+          }
+          if (fuse) fuse_single_qubit_gates();
+     }
+
+     std::vector<int64_t> run()
+     {
+          if (gate_.empty()) return {};
+          best_ans_ = 0;
+          best_ = 0;
+          rec(0, 0, 0, 0, num_splits_);
+          std::vector<int64_t> ids;
+          for (int i = 0; i < 64 && (msk_t(1) << i) <= best_; ++i)
+               if ((best_ >> i) & 1) ids.push_back(pos_.pos_to_id[i]);
+          return ids;
+     }
+
+private:
+     bool can_take(size_t i, msk_t cur, msk_t bad) const
+     {
+          if ((ctrl_[i] | gate_[i]) & bad) return false;
+          return diag_[i] ? true : popc(gate_[i] | cur) <= num_locals_;
+     }
+
+     // swap_scheduler.h:87-104 (FuseSingleQubitGateToThis).
+     void fuse_into(size_t i, size_t j)
+     {
+          weight_[j] += weight_[i];
+          weight_[i] = 0;
+          if (!diag_[i]) diag_[j] = false;
+          if (ctrl_[j] & gate_[i]) {
+               ctrl_[j] ^= gate_[i];
+               gate_[j] |= gate_[i];
+          }
+     }
+     bool fuse_prev(size_t i)
+     {
+          for (size_t j = i; j-- > 0;)
+               if (gate_[i] & (gate_[j] | ctrl_[j])) {
+                    fuse_into(i, j);
+                    return true;
+               }
+          return false;
+     }
+     bool fuse_next(size_t i)
+     {
+          for (size_t j = i + 1; j < gate_.size(); ++j)
+               if (gate_[i] & (gate_[j] | ctrl_[j])) {
+                    fuse_into(i, j);
+                    return true;
+               }
+          return false;
+     }
+     // swap_scheduler.cpp:47-68.
+     void fuse_single_qubit_gates()
+     {
+          for (size_t i = gate_.size(); i-- > 0;) {
+               bool remove = false;
+               if (popc(gate_[i] | ctrl_[i]) == 1)
+                    remove = !diag_[i] ? (fuse_next(i) || fuse_prev(i)) : (fuse_prev(i) || fuse_next(i));
+               if (remove) {
+                    gate_.erase(gate_.begin() + static_cast<long>(i));
+                    ctrl_.erase(ctrl_.begin() + static_cast<long>(i));
+                    diag_.erase(diag_.begin() + static_cast<long>(i));
+                    weight_.erase(weight_.begin() + static_cast<long>(i));
+               }
+          }
+     }
+
+     // swap_scheduler.cpp:90-170: take/skip backtracking with a split budget.
+     int64_t rec(size_t pos, msk_t cur, msk_t bad, int64_t ans, int64_t splits)
+     {
+          if (ans > best_ans_) {
+               best_ans_ = ans;
+               best_ = cur;
+          }
+          if (pos == gate_.size()) return splits;
+          bool can_skip = true;
+          int cnt = 1;
+          const bool can_take_now = can_take(pos, cur, bad);
+          if (can_take_now) {
+               if (!diag_[pos]) {
+                    if (submask(gate_[pos], cur)) {
+                         can_skip = false;
+                         --cnt;
+                    }
+                    ++cnt;
+               }
+               else {
+                    can_skip = false;
+               }
+          }
+          if (!(0 < cnt && cnt <= 2)) throw std::logic_error("SwapScheduler: branch count");
+          if (splits == 0 && can_take_now && can_skip) {
+               can_skip = false;
+               --cnt;
+          }
+          splits -= cnt - 1;
+          if (splits < 0) throw std::logic_error("SwapScheduler: split budget");
+          if (can_skip) {
+               const int64_t give = splits / cnt;
+               splits += rec(pos + 1, cur, bad | gate_[pos] | ctrl_[pos], ans, give) - give;
+               --cnt;
+          }
+          if (can_take_now) {
+               const msk_t nl = diag_[pos] ? cur : (cur | gate_[pos]);
+               splits = rec(pos + 1, nl, bad, ans + weight_[pos], splits);
+               --cnt;
+          }
+          if (cnt != 0) throw std::logic_error("SwapScheduler: invalid branch number");
+          return splits;
+     }
+
+     int64_t num_splits_;
+     int num_locals_;
+     std::vector<bool> diag_;
+     std::vector<int64_t> weight_;
+     Positions pos_;
+     std::vector<msk_t> gate_, ctrl_;
+     int64_t best_ans_ = 0;
+     msk_t best_ = 0;
+};
+
+}  // namespace
+
+std::vector<int> schedule_cluster(const GateList& g, const std::vector<int64_t>& locals,
+                                  const std::vector<int64_t>& globals, int cluster_size)
+{
+     Cluster c(g, locals, globals, cluster_size);
+     return c.run();
+}
+
+std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals, bool fuse)
+{
+     Swap s(g, num_splits, num_locals, fuse);
+     return s.run();
+}
+
+}  // namespace hq

@@ -1,0 +1,44 @@
+// Gate schedulers (host, integer bitmask work).
+//
+// ClusterScheduler picks the next fused cluster; SwapScheduler picks the next
+// stage's local qubit set.  Their selection rules are those of the
+// reference's src/scheduler (cluster_scheduler.{h,cpp}, swap_scheduler.{h,cpp},
+// convertors.cpp), kept identical because the Python GreedyScheduler drives
+// them through the same interface (_sched_cpp.cpp:28-40,
+// _greedyscheduler.py:119-193) and the order of the returned gate indices is
+// observable:
+//   * qubit ids are renumbered densely in ascending order (< 64 qubits);
+//   * a gate can join a cluster if it touches no qubit of an earlier skipped
+//     gate and its local qubits lie in the cluster; a non-diagonal gate must
+//     be fully local;
+//   * the best cluster maximises the number of takeable gates, ties go to
+//     fewer qubits, then to the first candidate in enumeration order.
+// The candidate enumeration (supersets of each gate's local set, adding bits
+// in ascending order, each (cluster, next bit) pair expanded once) is kept;
+// the memo is an open-addressing table instead of std::map and a candidate's
+// scan stops once every cluster qubit is blocked and no later gate could
+// still be taken, which leaves every result unchanged.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace hq {
+
+using msk_t = uint64_t;
+
+struct GateList {
+     std::vector<std::vector<int64_t>> gate;
+     std::vector<std::vector<int64_t>> ctrl;
+     std::vector<bool> diag;
+};
+
+// cluster_scheduler.cpp:42-78.  Returns indices into the gate list.
+std::vector<int> schedule_cluster(const GateList& g, const std::vector<int64_t>& locals,
+                                  const std::vector<int64_t>& globals, int cluster_size);
+
+// swap_scheduler.cpp:70-88.  Returns the qubit ids to be local.
+std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals,
+                                   bool fuse);
+
+}  // namespace hq

@@ -130,6 +130,30 @@ int hiqsim_synchronize(hiqsim_t* h);
 int hiqsim_set_timing(hiqsim_t* h, int on);
 int hiqsim_kernel_time(hiqsim_t* h, int cls, uint64_t* count, double* total_ms);
 
+/* ---- gate schedulers (host only; no device needed) ----------------------
+ * Gate lists are ragged: gate g's target ids are gate_ids[gate_off[g] ..
+ * gate_off[g+1]), its control ids ctrl_ids[ctrl_off[g] .. ctrl_off[g+1]);
+ * gate_off and ctrl_off hold ngates+1 entries; diag[g] != 0 marks a diagonal
+ * gate.  At most 63 distinct qubit ids. */
+
+/* ClusterScheduler(gate, gate_ctrl, gate_diag, locals, globals,
+ * cluster_size).ScheduleCluster() (_sched_cpp.cpp:36-39,
+ * cluster_scheduler.cpp:42-99): indices of the gates forming the next fused
+ * cluster, in gate order.  out must hold ngates entries; *nout is set. */
+int hiqsim_schedule_cluster(const int64_t* gate_ids, const size_t* gate_off,
+                            const int64_t* ctrl_ids, const size_t* ctrl_off, const uint8_t* diag,
+                            size_t ngates, const int64_t* locals, size_t nlocals,
+                            const int64_t* globals, size_t nglobals, int cluster_size,
+                            int32_t* out, size_t* nout);
+/* SwapScheduler(gate, gate_ctrl, gate_diag, num_splits, num_locals,
+ * fuse).ScheduleSwap() (_sched_cpp.cpp:31-35, swap_scheduler.cpp:25-170):
+ * the qubit ids to hold locally in the next stage, ascending.  out must hold
+ * 64 entries; *nout is set. */
+int hiqsim_schedule_swap(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, const uint8_t* diag, size_t ngates,
+                         int64_t num_splits, int num_locals, int fuse, int64_t* out,
+                         size_t* nout);
+
 #ifdef __cplusplus
 }
 #endif

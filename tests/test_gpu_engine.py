@@ -73,7 +73,7 @@ def test_fusion_with_folded_controls(engine_mod, oracle_mod):
             s.apply_controlled_gate(H, [q], [])
             s.run()
         for item in seq:
-            if item[0] == "run":
+            if isinstance(item[0], str):
                 s.run()
             else:
                 s.apply_controlled_gate(*item)
