@@ -153,6 +153,18 @@ def cpu_baseline(k, seconds=12.0, n_sample=28):
                       % (passes, k, n_sample, el)}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE/WRITE_SIZE, calibrated on a plain copy)."""
+    path = os.path.join(ROOT, "profiles", "dense_pass_30q_c128_hbm_traffic.md")
+    if kernel != "dense_k4" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(line.split(":")[1].split()[0])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +177,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
@@ -184,6 +198,10 @@ def main():
     sim = hq.SimulatorMPI(args.seed, n, k)
     sim.allocate_qureg(list(range(n)), 0)
     sim.synchronize()
+    if args.calibrate_copy:
+        sim.snapshot()
+        sim.synchronize()
+        log("calibration copy: %d bytes read + %d bytes written" % (16 * 2 ** n, 16 * 2 ** n))
     for w in range(args.warmup):
         run_step(sim, step)
         sim.synchronize()
@@ -237,7 +255,8 @@ def main():
                    "permutations_per_step": n_perm, "plan_seconds": round(plan_s, 3),
                    "final_norm": norm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom) if n == 30 else None,
                      "kernel": dom, "launches": cnt, "avg_launch_ms": round(per_launch_ms, 4)},
         "kernel_ms_total": kernel_share,
         "k_sweep_gbs": sweep,

@@ -297,6 +297,22 @@ int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps)
      });
 }
 
+int hiqsim_snapshot_state(hiqsim_t* h)
+{
+     return guard([&] {
+          need(h, "handle");
+          h->eng.snapshot();
+     });
+}
+
+int hiqsim_restore_state(hiqsim_t* h)
+{
+     return guard([&] {
+          need(h, "handle");
+          h->eng.restore();
+     });
+}
+
 int hiqsim_expectation_pauli(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
                              const double* coeff, size_t nterms, const int64_t* ids, size_t nids,
                              double* out)

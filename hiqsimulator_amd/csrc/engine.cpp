@@ -93,6 +93,7 @@ Engine::~Engine()
           if (ring_ev_[i]) (void)hipEventDestroy(ring_ev_[i]);
      if (d_state_) (void)hipFree(d_state_);
      if (d_tmp_) (void)hipFree(d_tmp_);
+     if (d_snap_) (void)hipFree(d_snap_);
      if (d_ring_) (void)hipFree(d_ring_);
      if (h_ring_) (void)hipHostFree(h_ring_);
      if (d_scratch_) (void)hipFree(d_scratch_);
@@ -679,6 +680,49 @@ void Engine::set_state_c128(const std::complex<double>* in, uint64_t n)
      check(hipMemcpyAsync(d_state_, tmp.data(), n * 8, hipMemcpyHostToDevice, stream_),
            "hipMemcpyAsync(state)");
      check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+// ---------------------------------------------------------------------------
+// Device-side checkpoint
+// ---------------------------------------------------------------------------
+void Engine::snapshot()
+{
+     DeviceScope ds(device_);
+     if (n_amps_ > cap_snap_) {
+          if (d_snap_) {
+               check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+               check(hipFree(d_snap_), "hipFree");
+               d_snap_ = nullptr;
+               cap_snap_ = 0;
+          }
+          check(hipMalloc(&d_snap_, n_amps_ * amp_bytes(prec_)), "hipMalloc(snapshot)");
+          cap_snap_ = n_amps_;
+     }
+     begin_timed(kFillCopy);
+     check(launch_copy(d_snap_, d_state_, prec_, n_amps_, stream_), "copy");
+     end_timed();
+     n_snap_ = n_amps_;
+     snap_locals_ = locals_;
+     has_snap_ = true;
+}
+
+void Engine::restore()
+{
+     DeviceScope ds(device_);
+     if (!has_snap_) throw std::runtime_error("restore(): no snapshot taken");
+     if (n_snap_ > cap_state_) {
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          check(hipFree(d_state_), "hipFree");
+          d_state_ = nullptr;
+          cap_state_ = 0;
+          check(hipMalloc(&d_state_, n_snap_ * amp_bytes(prec_)), "hipMalloc(state)");
+          cap_state_ = n_snap_;
+     }
+     begin_timed(kFillCopy);
+     check(launch_copy(d_state_, d_snap_, prec_, n_snap_, stream_), "copy");
+     end_timed();
+     n_amps_ = n_snap_;
+     locals_ = snap_locals_;
 }
 
 // ---------------------------------------------------------------------------

@@ -83,6 +83,13 @@ public:
      void get_state_c128(std::complex<double>* out);
      void set_state_c128(const std::complex<double>* in, uint64_t n);
 
+     // Device-side checkpoint: snapshot() copies the state (and the qubit
+     // order) into a second HBM buffer with one streaming copy; restore()
+     // copies it back.  No counterpart in the reference, which can only copy
+     // the state out (cheat_local).
+     void snapshot();
+     void restore();
+
      // Pauli sums; mask bit i refers to ids[i].  Y is encoded as X and Z.
      cplx expectation(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
                       const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
@@ -135,6 +142,10 @@ private:
      uint64_t cap_state_ = 0;  // amplitudes
      void* d_tmp_ = nullptr;
      uint64_t cap_tmp_ = 0;
+     void* d_snap_ = nullptr;
+     uint64_t cap_snap_ = 0, n_snap_ = 0;
+     std::vector<int64_t> snap_locals_;
+     bool has_snap_ = false;
      uint64_t n_amps_ = 1;
 
      // Fused-matrix upload ring: pinned host slots copied into device slots.

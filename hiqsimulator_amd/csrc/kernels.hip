@@ -1,13 +1,20 @@
 // gfx950 (CDNA4) kernels of the state-vector hot path.
 //
 // Every kernel is one streaming pass over the state in HBM; the bound is HBM
-// bandwidth (see launch_* comments for the algorithmic bytes).  Threads map to
-// "groups" (the 2^k amplitudes one fused matrix mixes) so that consecutive
-// lanes of a wave take consecutive group bases: every global load and store of
-// a wave then covers contiguous 16 B (complex128) or 8 B (complex64) slots
-// whenever the low index bits are not targets.  The fused matrix is uniform
-// across the grid and is read with scalar loads into SGPRs, which v_fma_f64 /
-// v_fma_f32 take directly as an operand.
+// bandwidth (algorithmic bytes: one read and one write of each amplitude a
+// pass touches).  The fused k-qubit pass has two forms:
+//  * wave-tile (default): a wave owns 64 groups (the 2^k amplitudes one
+//    fused matrix mixes, one group per lane).  Their 64 * 2^k amplitudes are
+//    read as 2^k fully contiguous 64-amplitude chunks (1 KiB for complex128)
+//    whatever the target bits are, regrouped through LDS, multiplied in
+//    registers, regrouped back and written as the same chunks.  Global
+//    accesses are non-temporal (the state is streamed once per pass).
+//  * direct: each lane loads its own group straight from HBM; used when the
+//    target bits are all high (the chunks of a tile then lie far apart and
+//    the LDS round trip costs more than it saves) or when a low control bit
+//    breaks the chunk layout.
+// The fused matrix is uniform across the grid and is read with scalar loads
+// into SGPRs, which v_fma_f64 / v_fma_f32 take directly as an operand.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -31,6 +38,9 @@ struct C2<float> {
      using T = float2;
 };
 
+template <typename R>
+using VT = typename C2<R>::T;
+
 __device__ __forceinline__ uint64_t insert_zeros(uint64_t t, const InsertSet& s)
 {
      for (int i = 0; i < s.n; ++i) {
@@ -48,17 +58,91 @@ inline unsigned grid_for(uint64_t work_items, uint64_t per_block)
      return static_cast<unsigned>(g);
 }
 
+// Non-temporal 16-byte / 8-byte global accesses for streamed amplitudes.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double2 ld_nt(const double2* p)
+{
+     const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+     return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ld_nt(const float2* p)
+{
+     const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+     return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 a)
+{
+     const d2v v = {a.x, a.y};
+     __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(p));
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 a)
+{
+     const f2v v = {a.x, a.y};
+     __builtin_nontemporal_store(v, reinterpret_cast<f2v*>(p));
+}
+template <bool NT, typename V>
+__device__ __forceinline__ V ld(const V* p)
+{
+     return NT ? ld_nt(p) : *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void st(V* p, V a)
+{
+     if (NT) st_nt(p, a);
+     else *p = a;
+}
+
+// out_i = sum_j m[i][j] v_j over one group held in registers; `put(i, o)`
+// stores output i.  The matrix is read row by row with uniform loads.
+template <int K, typename V, typename R, typename Put>
+__device__ __forceinline__ void group_matvec(const V* __restrict__ mat, const V (&v)[1 << K], Put&& put)
+{
+     constexpr int D = 1 << K;
+#pragma unroll
+     for (int i = 0; i < D; ++i) {
+          R re = 0, im = 0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+               const V m = mat[i * D + j];
+               re = fma(m.x, v[j].x, re);
+               re = fma(-m.y, v[j].y, re);
+               im = fma(m.x, v[j].y, im);
+               im = fma(m.y, v[j].x, im);
+          }
+          V o;
+          o.x = re;
+          o.y = im;
+          put(i, o);
+     }
+}
+
+__device__ __forceinline__ uint64_t block_linear()
+{
+     return static_cast<uint64_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+}
+
+// Grid covering `blocks` blocks exactly (a power of two or not) with a 2-D
+// launch so that very large states stay within the 32-bit dispatch limits.
+inline dim3 grid_2d(uint64_t blocks)
+{
+     if (blocks == 0) blocks = 1;
+     const uint64_t gx = blocks < 65536 ? blocks : 65536;
+     return dim3(static_cast<unsigned>(gx), static_cast<unsigned>((blocks + gx - 1) / gx));
+}
+
 // ---------------------------------------------------------------------------
-// Dense fused k-qubit pass.  Restates kernels/intrin/kernel{1..5}.hpp
-// (kernel_core + kernel_compute): for each base I with target bits 0 and
-// control bits 1, v_j = psi[I + off_j], psi[I + off_i] = sum_j m[i][j] v_j.
-// GPT groups per thread are loaded before any is computed so that
-// GPT * 2^K independent loads are in flight per lane.
+// Dense fused k-qubit pass, direct form.  Restates kernels/intrin/
+// kernel{1..5}.hpp (kernel_core + kernel_compute): for each base I with
+// target bits 0 and control bits 1, v_j = psi[I + off_j],
+// psi[I + off_i] = sum_j m[i][j] v_j.  GPT groups per lane are loaded before
+// any is computed so that GPT * 2^K loads are in flight per lane.
 // ---------------------------------------------------------------------------
-template <int K, typename R, int GPT>
+template <int K, typename R, int GPT, bool NT>
 __global__ void __launch_bounds__(kBlock)
-    k_dense(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
-            const DenseDesc d, const uint64_t ngroups)
+    k_dense_direct(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
+                   const DenseDesc d, const uint64_t ngroups)
 {
      using V = typename C2<R>::T;
      constexpr int D = 1 << K;
@@ -71,42 +155,84 @@ __global__ void __launch_bounds__(kBlock)
                if ((j >> l) & 1) o += 1ull << d.pos[l];
           off[j] = o;
      }
-     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * (kBlock * GPT);
-     for (uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * (kBlock * GPT) + threadIdx.x;
-          t0 < ngroups; t0 += stride) {
-          uint64_t I[GPT];
-          V v[GPT][D];
+     const uint64_t t0 = block_linear() * (kBlock * GPT) + threadIdx.x;
+     uint64_t I[GPT];
+     V v[GPT][D];
 #pragma unroll
-          for (int g = 0; g < GPT; ++g) {
-               const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
-               I[g] = insert_zeros(t < ngroups ? t : 0, d.ins) | d.ctrlmask;
-               if (t < ngroups) {
+     for (int g = 0; g < GPT; ++g) {
+          const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
+          I[g] = insert_zeros(t < ngroups ? t : 0, d.ins) | d.ctrlmask;
+          if (t < ngroups) {
 #pragma unroll
-                    for (int j = 0; j < D; ++j) v[g][j] = psi[I[g] + off[j]];
-               }
-          }
-#pragma unroll
-          for (int g = 0; g < GPT; ++g) {
-               const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
-               if (t >= ngroups) continue;
-#pragma unroll
-               for (int i = 0; i < D; ++i) {
-                    R re = 0, im = 0;
-#pragma unroll
-                    for (int j = 0; j < D; ++j) {
-                         const V m = mat[i * D + j];
-                         re = fma(m.x, v[g][j].x, re);
-                         re = fma(-m.y, v[g][j].y, re);
-                         im = fma(m.x, v[g][j].y, im);
-                         im = fma(m.y, v[g][j].x, im);
-                    }
-                    V o;
-                    o.x = re;
-                    o.y = im;
-                    psi[I[g] + off[i]] = o;
-               }
+               for (int j = 0; j < D; ++j) v[g][j] = ld<NT>(psi + I[g] + off[j]);
           }
      }
+#pragma unroll
+     for (int g = 0; g < GPT; ++g) {
+          if (t0 + static_cast<uint64_t>(g) * kBlock >= ngroups) continue;
+          V* const p = psi + I[g];
+          group_matvec<K, V, R>(mat, v[g], [&](int i, V o) { st<NT>(p + off[i], o); });
+     }
+}
+
+// Wave-tile description (built on the host).  The tile index r of an
+// amplitude is its index bits at the tile positions V = (targets) + (the six
+// lowest non-inserted positions), in ascending bit order; V always holds
+// bits 0..5 here, so amplitude r of a wave's tile sits at
+// base + (r & 63) + hoff[r >> 6].
+struct TileDesc {
+     uint64_t hoff[32];   // address offset of chunk i = pdep(i, V above bit 5)
+     uint8_t rank_n[6];   // tile rank of the group index's six lowest bits
+     uint8_t rank_t[5];   // tile rank of matrix bit l
+};
+
+// LDS slot of tile element r: XOR-swizzled so that both the chunk-order
+// accesses (consecutive r across lanes) and the group-order accesses (lanes
+// strided by powers of two) spread over the banks.
+template <typename V>
+__device__ __forceinline__ int tile_slot(int r)
+{
+     constexpr int S = sizeof(V) == 16 ? 4 : 5;
+     return r ^ ((r >> S) & ((1 << S) - 1));
+}
+
+template <int K, typename R, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_dense_tile(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
+                 const DenseDesc d, const TileDesc L)
+{
+     using V = typename C2<R>::T;
+     constexpr int D = 1 << K;
+     __shared__ V tile[WPB][64 * D];
+     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+     V* const T = tile[wv];
+     const uint64_t wave = block_linear() * WPB + wv;
+     V* const base = psi + (insert_zeros(wave << 6, d.ins) | d.ctrlmask) + lane;
+     V v[D];
+#pragma unroll
+     for (int i = 0; i < D; ++i) v[i] = ld<true>(base + L.hoff[i]);
+#pragma unroll
+     for (int i = 0; i < D; ++i) T[tile_slot<V>(i * 64 + lane)] = v[i];
+     __syncthreads();
+     int rl = 0;
+#pragma unroll
+     for (int b = 0; b < 6; ++b) rl |= ((lane >> b) & 1) << L.rank_n[b];
+     int slot[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          int r = rl;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) r |= 1 << L.rank_t[l];
+          slot[j] = tile_slot<V>(r);
+     }
+#pragma unroll
+     for (int j = 0; j < D; ++j) v[j] = T[slot[j]];
+     __syncthreads();
+     group_matvec<K, V, R>(mat, v, [&](int i, V o) { T[slot[i]] = o; });
+     __syncthreads();
+#pragma unroll
+     for (int i = 0; i < D; ++i) st<true>(base + L.hoff[i], T[tile_slot<V>(i * 64 + lane)]);
 }
 
 template <typename R>
@@ -430,19 +556,84 @@ __global__ void __launch_bounds__(kBlock)
      out[b] = s;
 }
 
-template <typename R>
-using VT = typename C2<R>::T;
+// Waves per block of the tile kernel: LDS holds 64 * 2^K amplitudes per wave.
+template <int K, typename R>
+constexpr int tile_wpb()
+{
+     return (64 << K) * static_cast<int>(sizeof(typename C2<R>::T)) > 16384 ? 2 : 4;
+}
+
+// Target bits at or above this position make the direct form faster than
+// the tile form (their chunks lie >= 2^g_high_target amplitudes apart).
+int g_high_target = 20;
 
 template <int K, typename R>
 hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStream_t s)
 {
-     constexpr int GPT = K >= 4 ? 1 : (K == 3 ? 2 : 4);
+     using V = VT<R>;
      const int nins = d.ins.n;
      if (nins > n) return hipErrorInvalidValue;
      const uint64_t ngroups = 1ull << (n - nins);
-     const unsigned grid = grid_for(ngroups, static_cast<uint64_t>(kBlock) * GPT);
-     hipLaunchKernelGGL((k_dense<K, R, GPT>), dim3(grid), dim3(kBlock), 0, s,
-                        static_cast<VT<R>*>(psi), static_cast<const VT<R>*>(mat), d, ngroups);
+     uint64_t ins_mask = 0, tmask = 0;
+     for (int i = 0; i < nins; ++i) ins_mask |= 1ull << d.ins.p[i];
+     int min_t = 64;
+     for (int l = 0; l < K; ++l) {
+          tmask |= 1ull << d.pos[l];
+          min_t = d.pos[l] < static_cast<uint32_t>(min_t) ? static_cast<int>(d.pos[l]) : min_t;
+     }
+     constexpr int WPB = tile_wpb<K, R>();
+     // Tile form: bits 0..5 must all be targets or free (no control among
+     // them) and there must be at least one full block of waves.
+     const bool low_ctrl = ((ins_mask & ~tmask) & 63ull) != 0;
+     const bool tileable = !low_ctrl && ngroups >= 64ull * WPB;
+     // For k = 5 the tile holds 32 KiB (complex128) per wave, which caps the
+     // occupancy; the direct form is faster there unless bit 0 (or, for
+     // complex64, bit 0 or 1) is a target.
+     const bool tile_pays = K < 5 || min_t <= (sizeof(V) == 16 ? 0 : 1);
+     if (tileable && tile_pays && min_t < g_high_target) {
+          TileDesc L{};
+          uint64_t lanes = 0;
+          int nb = 0;
+          for (int p = 0; p < n && nb < 6; ++p)
+               if (!((ins_mask >> p) & 1)) {
+                    lanes |= 1ull << p;
+                    ++nb;
+               }
+          const uint64_t V_bits = lanes | tmask;
+          int vpos[64], nv = 0, rank_of[64];
+          for (int p = 0; p < 64; ++p)
+               if ((V_bits >> p) & 1) {
+                    rank_of[p] = nv;
+                    vpos[nv++] = p;
+               }
+          if (nv != 6 + K) return hipErrorInvalidValue;
+          nb = 0;
+          for (int p = 0; p < 64; ++p)
+               if ((lanes >> p) & 1) L.rank_n[nb++] = static_cast<uint8_t>(rank_of[p]);
+          for (int l = 0; l < K; ++l) L.rank_t[l] = static_cast<uint8_t>(rank_of[d.pos[l]]);
+          for (int i = 0; i < (1 << K); ++i) {
+               uint64_t o = 0;
+               for (int b = 0; b < K; ++b)
+                    if ((i >> b) & 1) o |= 1ull << vpos[6 + b];
+               L.hoff[i] = o;
+          }
+          const uint64_t blocks = ngroups / (64ull * WPB);
+          hipLaunchKernelGGL((k_dense_tile<K, R, WPB>), grid_2d(blocks), dim3(64 * WPB), 0, s,
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, L);
+          return hipGetLastError();
+     }
+     // Direct form; non-temporal when every lane access covers whole 128-byte
+     // lines (no inserted bit below 128 / sizeof(V) amplitudes).
+     constexpr int GPT = K >= 4 ? 1 : (K == 3 ? 4 : (K == 2 ? 8 : 16));
+     const int line_bits = sizeof(V) == 16 ? 3 : 4;
+     const bool nt = (ins_mask & ((1ull << line_bits) - 1)) == 0;
+     const dim3 grid = grid_2d((ngroups + kBlock * GPT - 1) / (kBlock * GPT));
+     if (nt)
+          hipLaunchKernelGGL((k_dense_direct<K, R, GPT, true>), grid, dim3(kBlock), 0, s,
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups);
+     else
+          hipLaunchKernelGGL((k_dense_direct<K, R, GPT, false>), grid, dim3(kBlock), 0, s,
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups);
      return hipGetLastError();
 }
 
@@ -510,6 +701,8 @@ hipError_t reduce_launch(uint64_t work, double* scratch, double* out, hipStream_
 }
 
 }  // namespace
+
+void set_tile_high_target(int pos) { g_high_target = pos; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -38,6 +38,10 @@ struct DenseDesc {
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat,
                         const DenseDesc& d, hipStream_t s);
 
+// The dense pass takes its direct form (instead of the LDS tile form) when
+// every target bit is at or above this position (default 20).
+void set_tile_high_target(int pos);
+
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).
 hipError_t launch_diag(void* psi, Prec prec, int n_qubits, const double* diag_re_im,

@@ -279,6 +279,16 @@ public:
           ok(hiqsim_apply_matrix(h_, d, static_cast<int>(pos.size()), pos.data(), ctrlmask,
                                  diag ? 1 : 0));
      }
+     void snapshot()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_snapshot_state(h_));
+     }
+     void restore()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_restore_state(h_));
+     }
      void synchronize()
      {
           py::gil_scoped_release r;
@@ -421,6 +431,8 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("apply_qubit_operator", &PySim::apply_qubit_operator)
          .def("apply_matrix", &PySim::apply_matrix, py::arg("matrix"), py::arg("pos"),
               py::arg("ctrlmask") = 0, py::arg("diag") = false)
+         .def("snapshot", &PySim::snapshot)
+         .def("restore", &PySim::restore)
          .def("synchronize", &PySim::synchronize)
          .def("set_timing", &PySim::set_timing)
          .def("kernel_time", &PySim::kernel_time)

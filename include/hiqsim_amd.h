@@ -104,6 +104,13 @@ int hiqsim_get_state(hiqsim_t* h, double* out, uint64_t n_amps);
  * reference): replace the amplitudes, qubit order unchanged. */
 int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps);
 
+/* Device-side checkpoint (no counterpart in the reference, whose only state
+ * export is the cheat_local copy-out, SimulatorMPI.cpp:543-559): snapshot
+ * copies the state and qubit order into a second HBM buffer with one
+ * streaming copy; restore copies them back. */
+int hiqsim_snapshot_state(hiqsim_t* h);
+int hiqsim_restore_state(hiqsim_t* h);
+
 /* get_expectation_value (_simulator_mpi.py:148-183; unbound in the
  * reference): sum_t coeff[t] <psi|P_t|psi>.  Bit i of xmask[t]/zmask[t] refers
  * to ids[i]; X = x only, Z = z only, Y = both.  coeff and out are {re, im}. */

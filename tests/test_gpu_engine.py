@@ -219,3 +219,18 @@ def test_large_state_roundtrip_26q(engine_mod):
     apply_circuit_inorder(g, inv, 5)
     assert abs(g.get_amplitude([0] * n, list(range(n))) - 1.0) < 1e-10
     assert abs(g.get_probability([0] * n, list(range(n))) - 1.0) < 1e-10
+
+
+def test_snapshot_restore(engine_mod):
+    n = 18
+    g = make_engine(1, n, 4)
+    g.allocate_qureg(list(range(n)), 0)
+    apply_circuit_inorder(g, random_layers_circuit(n, 3, seed=4), 4)
+    before = g.get_state()
+    g.snapshot()
+    apply_circuit_inorder(g, random_layers_circuit(n, 2, seed=5), 4)
+    g.permute_qubits(list(reversed(range(n))))
+    assert not np.array_equal(g.get_state(), before)
+    g.restore()
+    assert g.get_qubits_ids() == list(range(n))
+    assert np.array_equal(g.get_state(), before)

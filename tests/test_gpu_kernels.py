@@ -112,3 +112,28 @@ def test_permutation_bit_exact(engine_mod, n, precision):
     expect[idx] = st
     assert np.array_equal(got, expect)
     assert sim.get_qubits_ids() == new
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_dense_forms_22q(engine_mod, oracle_mod, precision):
+    """Exercises both forms of the dense pass on a 22-qubit state: the LDS
+    tile form (targets low or mixed), the direct form (all targets >= 20)
+    and the direct form forced by a control bit below 6."""
+    n = 22
+    rng = np.random.default_rng(22)
+    st = rand_state(rng, n)
+    cases = [([20], 0), ([21, 20], 0), ([0, 1, 2, 3, 4], 0), ([5, 21, 0], 0), ([20, 21], 1 << 3),
+             ([7, 13], 1 << 2), ([1, 0, 9, 4], (1 << 20) | (1 << 15)), ([21], (1 << 0) | (1 << 19))]
+    tol = 1e-12 if precision == "c128" else 3e-6
+    for pos, cm in cases:
+        print("case pos=%s ctrl=%#x" % (pos, cm), flush=True)
+        k = len(pos)
+        m = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        m /= np.linalg.norm(m, 2)
+        sim = engine_with_state(engine_mod, st, precision)
+        sim.apply_matrix(m, pos, cm, False)
+        got = sim.get_state()
+        ref = st.astype(np.complex64).astype(np.complex128) if precision == "c64" else st.copy()
+        oracle_mod.dense_pass(ref, m, pos, cm)
+        err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
+        assert err <= tol, (pos, cm, err)
