@@ -51,7 +51,13 @@ def plan_passes(gates, k):
     return passes
 
 
-def build_step(n, layers, k, seed, n_perm):
+def plan_greedy(gates, k, n):
+    """The GreedyScheduler loop: ClusterScheduler clusters, one pass each."""
+    from hiqsimulator_amd.scheduling import plan_clusters
+    return [[gates[i] for i in c] for c in plan_clusters(gates, list(range(n)), k)]
+
+
+def build_step(n, layers, k, seed, n_perm, planner="greedy"):
     from hiqsimulator_amd.circuits import random_layers_circuit, random_permutation
     rng = np.random.default_rng(seed + 1)
     gates = random_layers_circuit(n, layers, seed)
@@ -73,16 +79,17 @@ def build_step(n, layers, k, seed, n_perm):
             ops.append(("perm", perm))
     assert idx == len(gates), (idx, len(gates))
     # Fuse within each stretch between permutations.
+    plan = (lambda g: plan_greedy(g, k, n)) if planner == "greedy" else (lambda g: plan_passes(g, k))
     step = []
     stretch = []
     for kind, payload in ops:
         if kind == "gates":
             stretch.extend(payload)
         else:
-            step.extend(("pass", p) for p in plan_passes(stretch, k))
+            step.extend(("pass", p) for p in plan(stretch))
             stretch = []
             step.append(("perm", payload))
-    step.extend(("pass", p) for p in plan_passes(stretch, k))
+    step.extend(("pass", p) for p in plan(stretch))
     return gates, step
 
 
@@ -175,6 +182,8 @@ def main():
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--perms", type=int, default=8)
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--planner", choices=["greedy", "inorder"], default="greedy",
+                    help="greedy: the GreedyScheduler/ClusterScheduler loop; inorder: program order")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -188,7 +197,7 @@ def main():
     import hiqsimulator_amd as hq
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
-    gates, step = build_step(n, args.layers, k, args.seed, args.perms)
+    gates, step = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
     n_pass = sum(1 for s in step if s[0] == "pass")
     n_perm = sum(1 for s in step if s[0] == "perm")
     plan_s = time.perf_counter() - t0
@@ -249,8 +258,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded random circuit)",
         "config": {"workload": "%dq complex128 random circuit, %d layers, %d gates, fused k=%d "
-                               "(%d passes) + %d qubit permutations" %
-                               (n, args.layers, len(gates), k, n_pass, n_perm),
+                               "(%s planner, %d passes) + %d qubit permutations" %
+                               (n, args.layers, len(gates), k, args.planner, n_pass, n_perm),
+                   "planner": args.planner,
                    "qubits": n, "layers": args.layers, "fusion_k": k, "passes_per_step": n_pass,
                    "permutations_per_step": n_perm, "plan_seconds": round(plan_s, 3),
                    "final_norm": norm},

@@ -243,7 +243,7 @@ struct DiagVals {
 // Diagonal fused pass: kernels/intrin/kernels_diag.hpp:34-144.  Threads
 // enumerate the indices whose control bits are 1 (zero bits inserted at the
 // control positions, ctrlmask OR-ed in); d(i) gathers the target bits.
-template <int K, typename R>
+template <int K, typename R, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_diag(typename C2<R>::T* __restrict__ psi, const DiagVals<R> dv, const InsertSet cins,
            const uint64_t ctrlmask, const uint32_t pos0, const uint32_t pos1, const uint32_t pos2,
@@ -263,12 +263,12 @@ __global__ void __launch_bounds__(kBlock)
                uint32_t di = 0;
 #pragma unroll
                for (int l = 0; l < K; ++l) di |= static_cast<uint32_t>((i >> pos[l]) & 1ull) << l;
-               const V a = psi[i];
+               const V a = ld<NT>(psi + i);
                const R mr = dv.v[2 * di], mi = dv.v[2 * di + 1];
                V o;
                o.x = fma(a.x, mr, -a.y * mi);
                o.y = fma(a.x, mi, a.y * mr);
-               psi[i] = o;
+               st<NT>(psi + i, o);
           }
      }
 }
@@ -280,11 +280,11 @@ __global__ void __launch_bounds__(kBlock)
      using V = typename C2<R>::T;
      const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
      for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-          const V a = psi[i];
+          const V a = ld<true>(psi + i);
           V o;
           o.x = fma(a.x, fr, -a.y * fi);
           o.y = fma(a.x, fi, a.y * fr);
-          psi[i] = o;
+          st<true>(psi + i, o);
      }
 }
 
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(kBlock)
      o.x = vr;
      o.y = vi;
      for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
-          psi[i] = o;
+          st<true>(psi + i, o);
 }
 
 // Plain streaming copy in 16-byte words.
@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(kBlock)
      using V = typename C2<R>::T;
      const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
      for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-          V a = psi[i];
+          V a = ld<true>(psi + i);
           if ((i & msk) != val) {
                a.x = 0;
                a.y = 0;
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kBlock)
                a.x *= f;
                a.y *= f;
           }
-          psi[i] = a;
+          st<true>(psi + i, a);
      }
 }
 
@@ -370,6 +370,70 @@ __global__ void __launch_bounds__(kBlock)
           const uint64_t i = tab[0][j & 255] | tab[1][(j >> 8) & 255] | tab[2][(j >> 16) & 255]
                              | tab[3][(j >> 24) & 255] | tab[4][(j >> 32) & 255];
           dst[j] = src[i];
+     }
+}
+
+// Tiled qubit permutation (LDS transpose).  The tile bits U (dst index
+// positions, ascending) contain dst bits 0..5 and the dst positions of src
+// bits 0..5, so both sides of the tile are read/written as contiguous
+// 64-amplitude runs.  Tile element r (src order: bit a of r = src bit
+// src_sorted[a]) lands at tile slot q (dst order: bit a of q = dst bit
+// u_sorted[a]); q is r with its bits moved by qbit_of_rbit.
+struct PermTile {
+     InsertSet u;           // tile dst positions, ascending (= insert set)
+     uint8_t src_sorted[12];  // src positions of the tile bits, ascending
+     uint8_t qbit_of_rbit[12];
+     uint8_t src_pos[40];   // dst bit p <- src bit src_pos[p]
+     int nq;
+};
+
+template <typename V>
+__device__ __forceinline__ int perm_slot(int q)
+{
+     return q ^ ((q >> 4) & 15) ^ ((q >> 8) & 15);
+}
+
+template <typename R, int T>
+__global__ void __launch_bounds__(kBlock)
+    k_permute_tile(typename C2<R>::T* __restrict__ dst, const typename C2<R>::T* __restrict__ src,
+                   const PermTile P)
+{
+     using V = typename C2<R>::T;
+     constexpr int E = 1 << (T - 8);
+     __shared__ V tile[1 << T];
+     const uint64_t dbase = insert_zeros(block_linear(), P.u);
+     uint64_t sbase = 0;
+     for (int p = 0; p < P.nq; ++p)
+          if ((dbase >> p) & 1) sbase |= 1ull << P.src_pos[p];
+     const int tid = threadIdx.x;
+     uint64_t slo = 0, dlo = 0;
+     int qlo = 0;
+#pragma unroll
+     for (int b = 0; b < 8; ++b) {
+          const uint64_t bit = (tid >> b) & 1;
+          slo |= bit << P.src_sorted[b];
+          dlo |= bit << P.u.p[b];
+          qlo |= static_cast<int>(bit) << P.qbit_of_rbit[b];
+     }
+#pragma unroll
+     for (int e = 0; e < E; ++e) {
+          uint64_t shi = 0;
+          int qhi = 0;
+#pragma unroll
+          for (int b = 0; b < T - 8; ++b) {
+               const uint64_t bit = (e >> b) & 1;
+               shi |= bit << P.src_sorted[8 + b];
+               qhi |= static_cast<int>(bit) << P.qbit_of_rbit[8 + b];
+          }
+          tile[perm_slot<V>(qlo | qhi)] = ld<true>(src + sbase + slo + shi);
+     }
+     __syncthreads();
+#pragma unroll
+     for (int e = 0; e < E; ++e) {
+          uint64_t dhi = 0;
+#pragma unroll
+          for (int b = 0; b < T - 8; ++b) dhi |= static_cast<uint64_t>((e >> b) & 1) << P.u.p[8 + b];
+          st<true>(dst + dbase + dlo + dhi, tile[perm_slot<V>(e * kBlock + tid)]);
      }
 }
 
@@ -667,9 +731,16 @@ hipError_t diag_k(void* psi, int n, const double* dre, const DenseDesc& d, hipSt
      if (cins.n > n) return hipErrorInvalidValue;
      const uint64_t count = 1ull << (n - cins.n);
      const unsigned grid = grid_for(count, static_cast<uint64_t>(kBlock) * 4);
-     hipLaunchKernelGGL((k_diag<K, R>), dim3(grid), dim3(kBlock), 0, s, static_cast<VT<R>*>(psi),
-                        dv, cins, d.ctrlmask, d.pos[0], d.pos[1], d.pos[2], d.pos[3], d.pos[4],
-                        count);
+     // Non-temporal when the control bits leave whole 128-byte lines intact.
+     const uint64_t line_mask = sizeof(VT<R>) == 16 ? 7 : 15;
+     if ((d.ctrlmask & line_mask) == 0)
+          hipLaunchKernelGGL((k_diag<K, R, true>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<VT<R>*>(psi), dv, cins, d.ctrlmask, d.pos[0], d.pos[1],
+                             d.pos[2], d.pos[3], d.pos[4], count);
+     else
+          hipLaunchKernelGGL((k_diag<K, R, false>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<VT<R>*>(psi), dv, cins, d.ctrlmask, d.pos[0], d.pos[1],
+                             d.pos[2], d.pos[3], d.pos[4], count);
      return hipGetLastError();
 }
 
@@ -796,10 +867,56 @@ hipError_t launch_permute(void* dst, const void* src, Prec prec, int n_qubits,
                           const uint8_t* src_pos, hipStream_t s)
 {
      if (n_qubits > 40) return hipErrorInvalidValue;
+     const uint64_t n = 1ull << n_qubits;
+     if (n_qubits >= 12) {
+          // Tile bits: dst bits 0..5, the dst positions of src bits 0..5, then
+          // the lowest remaining dst bits up to 12.
+          int d_of[40];
+          for (int p = 0; p < n_qubits; ++p) d_of[src_pos[p]] = p;
+          uint64_t U = 63;
+          for (int b = 0; b < 6; ++b) U |= 1ull << d_of[b];
+          for (int p = 0; p < n_qubits && __builtin_popcountll(U) < 12; ++p) U |= 1ull << p;
+          const int T = __builtin_popcountll(U);
+          PermTile P{};
+          P.nq = n_qubits;
+          int rank_u[40];
+          for (int p = 0; p < n_qubits; ++p) {
+               P.src_pos[p] = src_pos[p];
+               if ((U >> p) & 1) {
+                    rank_u[p] = P.u.n;
+                    P.u.p[P.u.n++] = static_cast<uint8_t>(p);
+               }
+          }
+          uint64_t S = 0;
+          for (int p = 0; p < n_qubits; ++p)
+               if ((U >> p) & 1) S |= 1ull << src_pos[p];
+          int a = 0;
+          for (int b = 0; b < n_qubits; ++b)
+               if ((S >> b) & 1) {
+                    P.src_sorted[a] = static_cast<uint8_t>(b);
+                    P.qbit_of_rbit[a] = static_cast<uint8_t>(rank_u[d_of[b]]);
+                    ++a;
+               }
+          const dim3 grid = grid_2d(n >> T);
+#define HQ_PERM_TILE(RT, VT_, TT)                                                                 \
+     hipLaunchKernelGGL((k_permute_tile<RT, TT>), grid, dim3(kBlock), 0, s, static_cast<VT_*>(dst), \
+                        static_cast<const VT_*>(src), P)
+          if (prec == Prec::C128) {
+               if (T == 10) HQ_PERM_TILE(double, double2, 10);
+               else if (T == 11) HQ_PERM_TILE(double, double2, 11);
+               else HQ_PERM_TILE(double, double2, 12);
+          }
+          else {
+               if (T == 10) HQ_PERM_TILE(float, float2, 10);
+               else if (T == 11) HQ_PERM_TILE(float, float2, 11);
+               else HQ_PERM_TILE(float, float2, 12);
+          }
+#undef HQ_PERM_TILE
+          return hipGetLastError();
+     }
      InsertSet sp{};
      sp.n = n_qubits;
      for (int p = 0; p < n_qubits; ++p) sp.p[p] = src_pos[p];
-     const uint64_t n = 1ull << n_qubits;
      const unsigned grid = grid_for(n, kBlock * 8);
      if (prec == Prec::C128)
           hipLaunchKernelGGL((k_permute<double>), dim3(grid), dim3(kBlock), 0, s,

@@ -230,3 +230,46 @@ def test_qft_closed_form(oracle_mod):
         apply_circuit_inorder(o, qft_like_circuit(n), 4)
         rev = np.array([int(format(i, "0%db" % n)[::-1], 2) for i in range(1 << n)])
         assert np.abs(o.get_state()[rev] - np.fft.ifft(st, norm="ortho")).max() < 1e-13
+
+
+def test_greedy_cluster_plan_is_equivalent(oracle_mod):
+    """The GreedyScheduler loop (ClusterScheduler clusters, one run each)
+    reorders gates only past disjoint ones: same state as program order."""
+    from hiqsimulator_amd.circuits import apply_circuit
+    from hiqsimulator_amd.scheduling import apply_scheduled, plan_clusters
+    n = 10
+    gates = random_layers_circuit(n, 8, seed=21)
+    for k in (2, 3, 4, 5):
+        clusters = plan_clusters(gates, list(range(n)), k)
+        assert sorted(i for c in clusters for i in c) == list(range(len(gates)))
+        for c in clusters:
+            qs = set(q for i in c for q in list(gates[i][1]) + list(gates[i][2]))
+            assert len(qs) <= k and c == sorted(c)
+        a = oracle_mod.OracleSim(1, n, k)
+        b = oracle_mod.OracleSim(1, n, k)
+        for s in (a, b):
+            s.allocate_qureg(list(range(n)), 0)
+        apply_scheduled(a, gates, k)
+        apply_circuit(b, gates, flush_every=1)
+        assert np.abs(a.get_state() - b.get_state()).max() < 1e-12
+
+
+def test_cluster_scheduler_known_structure():
+    """Small hand-checked cases of the selection rule: most takeable gates,
+    ties to fewer qubits, a blocked qubit stops later gates on it."""
+    import hiqsimulator_amd as hq
+    # gates: H0, H1, CX(0->2), H3; cluster size 2 -> {0,1}: gates 0,1 (2 gates)
+    # vs {0,2}: gates 0,2 (2 gates) -> tie, same size, first found wins.
+    cs = hq.ClusterScheduler([[0], [1], [2], [3]], [[], [], [0], []], [False] * 4,
+                             [0, 1, 2, 3], [], 2)
+    assert cs.ScheduleCluster() == [0, 1]
+    # with size 3, {0,1,2} takes H0, H1, CX(0->2).
+    cs = hq.ClusterScheduler([[0], [1], [2], [3]], [[], [], [0], []], [False] * 4,
+                             [0, 1, 2, 3], [], 3)
+    assert cs.ScheduleCluster() == [0, 1, 2]
+    # a skipped gate on qubit 1 blocks later gates on qubit 1.
+    cs = hq.ClusterScheduler([[1], [0], [1]], [[2], [], []], [False] * 3, [0, 1, 2], [], 1)
+    assert cs.ScheduleCluster() == [1]
+    # SwapScheduler with every qubit local keeps all of them.
+    ss = hq.SwapScheduler([[0], [1], [2]], [[], [0], []], [False] * 3, 10 ** 6, 3, True)
+    assert ss.ScheduleSwap() == [0, 1, 2]
