@@ -313,6 +313,23 @@ int hiqsim_restore_state(hiqsim_t* h)
      });
 }
 
+int hiqsim_compare_snapshot(hiqsim_t* h, double* out)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(out, "out");
+          h->eng.compare_snapshot(out);
+     });
+}
+
+int hiqsim_init_random_state(hiqsim_t* h, uint64_t seed)
+{
+     return guard([&] {
+          need(h, "handle");
+          h->eng.init_random_state(seed);
+     });
+}
+
 int hiqsim_expectation_pauli(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
                              const double* coeff, size_t nterms, const int64_t* ids, size_t nids,
                              double* out)

@@ -725,6 +725,34 @@ void Engine::restore()
      locals_ = snap_locals_;
 }
 
+void Engine::compare_snapshot(double out[4])
+{
+     DeviceScope ds(device_);
+     if (!has_snap_ || n_snap_ != n_amps_)
+          throw std::runtime_error("compare_snapshot(): no snapshot of the current size");
+     begin_timed(kReduce);
+     check(launch_compare(d_state_, d_snap_, prec_, n_amps_, d_scratch_,
+                          d_scratch_ + kReduceScratchDoubles, stream_),
+           "compare");
+     end_timed();
+     read_out(h_out_, 4);
+     for (int i = 0; i < 4; ++i) out[i] = h_out_[i];
+}
+
+void Engine::init_random_state(uint64_t seed)
+{
+     DeviceScope ds(device_);
+     begin_timed(kFillCopy);
+     check(launch_random_state(d_state_, prec_, n_amps_, seed, stream_), "random_state");
+     end_timed();
+     const double norm = masked_norm(0, 0);
+     if (!(norm > 0)) throw std::runtime_error("init_random_state(): zero norm");
+     const double f = 1.0 / std::sqrt(norm);
+     begin_timed(kScale);
+     check(launch_scale(d_state_, prec_, n_amps_, f, 0.0, stream_), "scale");
+     end_timed();
+}
+
 // ---------------------------------------------------------------------------
 // Pauli sums
 // ---------------------------------------------------------------------------

@@ -89,6 +89,12 @@ public:
      // the state out (cheat_local).
      void snapshot();
      void restore();
+     // Compares the state with the snapshot: {Re, Im of <snapshot|state>,
+     // <snapshot|snapshot>, max |state - snapshot|}, accumulated in FP64.
+     void compare_snapshot(double out[4]);
+     // Fills the state with a normalised complex Gaussian random vector
+     // (Philox4x32-10 keyed by `seed`, one counter per amplitude).
+     void init_random_state(uint64_t seed);
 
      // Pauli sums; mask bit i refers to ids[i].  Y is encoded as X and Z.
      cplx expectation(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,

@@ -587,6 +587,107 @@ __global__ void __launch_bounds__(kBlock)
      block_store<2>(acc, partials, gridDim.x);
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011): counter-based, so amplitude i draws from
+// counter i independently of the launch shape.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+     constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+     constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+     for (int r = 0; r < 10; ++r) {
+          const uint64_t p0 = static_cast<uint64_t>(M0) * c.x;
+          const uint64_t p1 = static_cast<uint64_t>(M1) * c.z;
+          const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+          const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+          c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+          k.x += W0;
+          k.y += W1;
+     }
+     return c;
+}
+
+// psi[i] = (g0 + i g1) with g0, g1 ~ N(0, 1) from Box-Muller on the Philox
+// block of counter (i, seed_hi); the caller normalises afterwards.
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_random_gaussian(typename C2<R>::T* __restrict__ psi, const uint64_t n, const uint64_t seed)
+{
+     using V = typename C2<R>::T;
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     const uint2 key = make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const uint4 r = philox4x32_10(
+              make_uint4(static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 0x5eedu, 0u), key);
+          // 53-bit uniforms in (0, 1].
+          const double u0 = ((static_cast<uint64_t>(r.x) << 21 ^ r.y) + 1.0) * 0x1p-53;
+          const double u1 = ((static_cast<uint64_t>(r.z) << 21 ^ r.w) + 1.0) * 0x1p-53;
+          const double rad = sqrt(-2.0 * log(u0));
+          double sn, cs;
+          sincospi(2.0 * u1, &sn, &cs);
+          V o;
+          o.x = static_cast<R>(rad * cs);
+          o.y = static_cast<R>(rad * sn);
+          st<true>(psi + i, o);
+     }
+}
+
+// Comparison with a reference copy: out[0..1] = sum conj(ref[i]) psi[i],
+// out[2] = sum |ref[i]|^2, out[3] = max |psi[i] - ref[i]| (FP64).
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_compare(const typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ ref,
+              const uint64_t n, double* __restrict__ partials)
+{
+     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+          const double ar = psi[i].x, ai = psi[i].y, br = ref[i].x, bi = ref[i].y;
+          acc[0] += br * ar + bi * ai;
+          acc[1] += br * ai - bi * ar;
+          acc[2] += br * br + bi * bi;
+          const double dr = ar - br, di = ai - bi;
+          acc[3] = fmax(acc[3], sqrt(dr * dr + di * di));
+     }
+     // sums for 0..2, max for 3
+     __shared__ double red[4][kBlock / 64];
+     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+     for (int o = 0; o < 3; ++o) {
+          const double sum = wave_sum(acc[o]);
+          if (lane == 0) red[o][w] = sum;
+     }
+     double m = acc[3];
+     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+     if (lane == 0) red[3][w] = m;
+     __syncthreads();
+     if (threadIdx.x == 0) {
+          for (int o = 0; o < 3; ++o) {
+               double sum = 0;
+               for (int i = 0; i < kBlock / 64; ++i) sum += red[o][i];
+               partials[o * gridDim.x + blockIdx.x] = sum;
+          }
+          double mm = 0;
+          for (int i = 0; i < kBlock / 64; ++i) mm = fmax(mm, red[3][i]);
+          partials[3 * gridDim.x + blockIdx.x] = mm;
+     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_compare_final(const double* __restrict__ partials, const int nblocks, double* __restrict__ out)
+{
+     if (threadIdx.x != 0) return;
+     double s[3] = {0, 0, 0}, m = 0;
+     for (int i = 0; i < nblocks; ++i) {
+          for (int o = 0; o < 3; ++o) s[o] += partials[o * nblocks + i];
+          m = fmax(m, partials[3 * nblocks + i]);
+     }
+     out[0] = s[0];
+     out[1] = s[1];
+     out[2] = s[2];
+     out[3] = m;
+}
+
 // Measurement block sums (SimulatorMPI.cpp:817-839): one workgroup per
 // contiguous block when the block is long, one thread per block otherwise.
 template <typename R>
@@ -999,6 +1100,39 @@ hipError_t launch_pauli_expectation(const void* psi, Prec prec, uint64_t n, uint
                hipLaunchKernelGGL((k_pauli_exp<float>), dim3(grid), dim3(kBlock), 0, s,
                                   static_cast<const float2*>(psi), n, xmask, zmask, scratch);
      });
+}
+
+hipError_t launch_random_state(void* psi, Prec prec, uint64_t n, uint64_t seed, hipStream_t s)
+{
+     const unsigned grid = grid_for(n, kBlock * 4);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_random_gaussian<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(psi), n, seed);
+     else
+          hipLaunchKernelGGL((k_random_gaussian<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(psi), n, seed);
+     return hipGetLastError();
+}
+
+hipError_t launch_compare(const void* psi, const void* ref, Prec prec, uint64_t n, double* scratch,
+                          double* out, hipStream_t s)
+{
+     uint64_t g = (n + kBlock - 1) / kBlock;
+     if (g < 1) g = 1;
+     if (g > static_cast<uint64_t>(kReduceBlocks)) g = kReduceBlocks;
+     const int grid = static_cast<int>(g);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_compare<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<const double2*>(psi), static_cast<const double2*>(ref), n,
+                             scratch);
+     else
+          hipLaunchKernelGGL((k_compare<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<const float2*>(psi), static_cast<const float2*>(ref), n,
+                             scratch);
+     hipError_t e = hipGetLastError();
+     if (e != hipSuccess) return e;
+     hipLaunchKernelGGL(k_compare_final, dim3(1), dim3(kBlock), 0, s, scratch, grid, out);
+     return hipGetLastError();
 }
 
 hipError_t launch_block_sums(const void* psi, Prec prec, uint64_t n, uint64_t nb, double* out,

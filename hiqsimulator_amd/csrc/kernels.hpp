@@ -94,6 +94,13 @@ hipError_t launch_pauli_expectation(const void* psi, Prec prec, uint64_t n, uint
 hipError_t launch_block_sums(const void* psi, Prec prec, uint64_t n, uint64_t nb, double* out,
                              hipStream_t s);
 
+// psi[i] = complex Gaussian N(0,1) + i N(0,1) from Philox4x32-10 keyed by
+// `seed` with counter i (not normalised).
+hipError_t launch_random_state(void* psi, Prec prec, uint64_t n, uint64_t seed, hipStream_t s);
+// out = {Re, Im of sum conj(ref) psi, sum |ref|^2, max |psi - ref|}.
+hipError_t launch_compare(const void* psi, const void* ref, Prec prec, uint64_t n, double* scratch,
+                          double* out, hipStream_t s);
+
 // dst[i ^ x] += c * phase(i) * src[i] for every i (one Pauli term of a sum).
 hipError_t launch_pauli_accumulate(void* dst, const void* src, Prec prec, uint64_t n,
                                    uint64_t xmask, uint64_t zmask, double cre, double cim,

@@ -289,6 +289,24 @@ public:
           py::gil_scoped_release r;
           ok(hiqsim_restore_state(h_));
      }
+     py::dict compare_snapshot()
+     {
+          double out[4];
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_compare_snapshot(h_, out));
+          }
+          py::dict d;
+          d["overlap"] = std::complex<double>(out[0], out[1]);
+          d["ref_norm"] = out[2];
+          d["max_abs_diff"] = out[3];
+          return d;
+     }
+     void init_random_state(uint64_t seed)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_init_random_state(h_, seed));
+     }
      void synchronize()
      {
           py::gil_scoped_release r;
@@ -432,6 +450,8 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("apply_matrix", &PySim::apply_matrix, py::arg("matrix"), py::arg("pos"),
               py::arg("ctrlmask") = 0, py::arg("diag") = false)
          .def("snapshot", &PySim::snapshot)
+         .def("compare_snapshot", &PySim::compare_snapshot)
+         .def("init_random_state", &PySim::init_random_state)
          .def("restore", &PySim::restore)
          .def("synchronize", &PySim::synchronize)
          .def("set_timing", &PySim::set_timing)

@@ -110,6 +110,12 @@ int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps);
  * streaming copy; restore copies them back. */
 int hiqsim_snapshot_state(hiqsim_t* h);
 int hiqsim_restore_state(hiqsim_t* h);
+/* Precision report against the snapshot (no reference counterpart):
+ * out = {Re, Im of <snapshot|state>, <snapshot|snapshot>, max |state - snapshot|}. */
+int hiqsim_compare_snapshot(hiqsim_t* h, double* out);
+/* Normalised complex Gaussian random state (Philox4x32-10 keyed by seed, one
+ * counter per amplitude); no reference counterpart. */
+int hiqsim_init_random_state(hiqsim_t* h, uint64_t seed);
 
 /* get_expectation_value (_simulator_mpi.py:148-183; unbound in the
  * reference): sum_t coeff[t] <psi|P_t|psi>.  Bit i of xmask[t]/zmask[t] refers
