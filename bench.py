@@ -160,6 +160,73 @@ def cpu_baseline(k, seconds=12.0, n_sample=28):
                       % (passes, k, n_sample, el)}
 
 
+def bench_batch(hq, count, seed, log_fn, plain_sample=64):
+    """Many-circuit batch: ``count`` circuits, n ~ U{10..22}, depth ~ U{10..60},
+    a Haar 2-qubit unitary every 5th layer, all in one graph launch, complex64
+    and complex128; plan from the stored autotuner table when present."""
+    from hiqsimulator_amd.batching import (batch_workload, fidelity, load_tuning, make_batch,
+                                           one_at_a_time)
+    t0 = time.perf_counter()
+    circuits = batch_workload(count, seed)
+    log_fn("batch: %d circuits generated (%.1f s)" % (count, time.perf_counter() - t0))
+    out = {"workload": "%d circuits, n~U{10..22}, depth~U{10..60}, Haar 2q every 5th layer"
+                       % count, "gates": int(sum(len(g) for _, g in circuits))}
+    sample_idx = list(range(0, count, max(1, count // 64)))[:64]
+    states = {}
+    for prec in ("c64", "c128"):
+        tune = load_tuning(prec)
+        k_by_n = tune["k_by_n"] if tune else {}
+        wg_max = tune["wg_max_qubits"] if tune else 16
+        t0 = time.perf_counter()
+        b = make_batch(circuits, prec, k_by_n, wg_max)
+        build_s = time.perf_counter() - t0
+        b.run()
+        ms = min(b.run() for _ in range(3))
+        st = b.stats()
+        ab = 16 if prec == "c128" else 8
+        states[prec] = [b.get_state(i) for i in sample_idx]
+        out[prec] = {"circuits_per_s": round(count / (ms * 1e-3), 1), "batch_ms": round(ms, 3),
+                     "effective_gbs": round(st["amp_passes"] * 2 * ab / (ms * 1e-3) / 1e9, 1),
+                     "fused_passes": st["passes"], "wg_circuits": st["wg_circuits"],
+                     "lds_circuits": st["lds_circuits"], "layer_launches": st["layer_launches"],
+                     "arena_gib": round(st["arena_bytes"] / 2 ** 30, 2), "build_s": round(build_s, 2),
+                     "plan": "autotuned table" if tune else "default (k=4, wg_max_qubits=16)",
+                     "wg_max_qubits": wg_max}
+        log_fn("batch %s: %s" % (prec, out[prec]))
+        del b
+    fids = [fidelity(a, b) for a, b in zip(states["c128"], states["c64"])]
+    errs = [float(np.abs(a - b).max()) for a, b in zip(states["c128"], states["c64"])]
+    out["c64_vs_c128"] = {"sample": len(fids), "min_fidelity": min(fids), "max_abs": max(errs)}
+    sample = [circuits[i] for i in sample_idx[:plain_sample]]
+    plain_s = one_at_a_time(sample, "c64")
+    out["plain_one_at_a_time_c64"] = {"sample": len(sample),
+                                      "circuits_per_s": round(len(sample) / plain_s, 1)}
+    log_fn("batch: plain path %s" % out["plain_one_at_a_time_c64"])
+    return out
+
+
+def bench_serving(count, rates, seed, log_fn):
+    """Poisson request streams, n log-uniform in 12..28, depth ~ U{20..100},
+    1024 samples or 8 Pauli expectations each, complex64; the request loop
+    against one-circuit-at-a-time SimulatorMPI on the same stream."""
+    from hiqsimulator_amd.serving import make_server, run_plain, run_server, serving_workload
+    out = []
+    for rate in rates:
+        reqs = serving_workload(count, rate, seed)
+        server = make_server("c64", state_bytes=64 << 30, matrix_bytes=4 << 30)
+        srv, _ = run_server(server, reqs)
+        st = server.stats()
+        del server
+        log_fn("serving rate %g: server %s" % (rate, srv))
+        plain, _ = run_plain(reqs, "c64")
+        log_fn("serving rate %g: plain %s" % (rate, plain))
+        out.append({"rate_per_s": rate, "requests": count,
+                    "server": {k: round(v, 3) for k, v in srv.items()},
+                    "plain": {k: round(v, 3) for k, v in plain.items()},
+                    "peak_active": st["peak_active"], "steps": st["steps"]})
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE/WRITE_SIZE, calibrated on a plain copy)."""
@@ -186,6 +253,11 @@ def main():
                     help="greedy: the GreedyScheduler/ClusterScheduler loop; inorder: program order")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", default="20,60")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()
@@ -243,6 +315,11 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(k)
     if cpu:
         log("cpu baseline: %s" % cpu)
+    batch = None if args.no_batch else bench_batch(hq, args.batch_circuits, args.seed, log)
+    serving = None
+    if not args.no_serving:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        serving = bench_serving(args.serve_requests, rates, args.seed, log)
 
     line = {
         "metric": "30q fp64 fused-pass HBM GB/s vs peak; mixed-size circuits/s + p99 latency",
@@ -271,6 +348,8 @@ def main():
         "kernel_ms_total": kernel_share,
         "k_sweep_gbs": sweep,
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

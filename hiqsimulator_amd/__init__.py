@@ -22,9 +22,14 @@ device_count = _cppsim_amd.device_count
 # The reference's `_sched_cpp` classes (_sched_cpp.cpp:28-40).
 SwapScheduler = _cppsim_amd.SwapScheduler
 ClusterScheduler = _cppsim_amd.ClusterScheduler
+# Many independent circuits in one graph launch, and the request loop with
+# continuous batching (no reference counterparts).
+Batch = _cppsim_amd.Batch
+Server = _cppsim_amd.Server
 
 # Kernel classes reported by SimulatorMPI.kernel_time(cls).
 KERNEL_CLASSES = ("dense_k1", "dense_k2", "dense_k3", "dense_k4", "dense_k5", "diag", "scale",
                   "permute", "reduce", "normalize", "remove_bit", "fill_copy", "pauli")
 
-__all__ = ["SimulatorMPI", "SwapScheduler", "ClusterScheduler", "device_count", "KERNEL_CLASSES", "LIB_PATH"]
+__all__ = ["SimulatorMPI", "SwapScheduler", "ClusterScheduler", "Batch", "Server", "device_count",
+           "KERNEL_CLASSES", "LIB_PATH"]

@@ -1,0 +1,161 @@
+"""Batches of independent circuits: workload generator, builder and autotuner.
+
+A ``Batch`` (native, ``hiqsimulator_amd.Batch``) runs many circuits, each
+from |0...0> on its own qubits 0..n-1, in one graph launch.  Its plan has two
+knobs, both picked here by measurement:
+
+* the fusion width k per qubit count n (every fused pass is a dense
+  2^k x 2^k matrix over at most k qubits);
+* ``wg_max_qubits``: circuits of at most this many qubits are each owned by
+  one workgroup of a persistent launch (LDS-resident up to 128 KiB of
+  state), larger ones are advanced by layered launches with many workgroups
+  per state.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import Batch
+from .circuits import random_layers_circuit
+
+DEFAULT_WG_MAX_QUBITS = 16
+TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+
+
+def tuning_path(precision, arch="gfx950"):
+    return os.path.join(TUNING_DIR, "%s_batch_%s.json" % (arch, precision))
+
+
+def load_tuning(precision, arch="gfx950"):
+    """The stored autotuner result for this precision, or None."""
+    path = tuning_path(precision, arch)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    t["k_by_n"] = {int(n): int(k) for n, k in t["k_by_n"].items()}
+    return t
+
+
+def save_tuning(result, precision, arch="gfx950"):
+    os.makedirs(TUNING_DIR, exist_ok=True)
+    with open(tuning_path(precision, arch), "w") as f:
+        json.dump({"k_by_n": {str(n): k for n, k in sorted(result["k_by_n"].items())},
+                   "wg_max_qubits": result["wg_max_qubits"],
+                   "probe_ms": result.get("probe_ms", {})}, f, indent=1, sort_keys=True)
+
+
+def batch_workload(count, seed, n_range=(10, 22), depth_range=(10, 60), haar_every=5):
+    """``count`` random layered circuits with n ~ U{n_range}, depth ~ U{depth_range}
+    and a Haar-random 2-qubit unitary on a random pair after every
+    ``haar_every``-th layer.  Returns ``[(n, gates), ...]``."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    ds = rng.integers(depth_range[0], depth_range[1] + 1, size=count)
+    seeds = rng.integers(0, 2**31 - 1, size=count)
+    return [(int(n), random_layers_circuit(int(n), int(d), int(s), haar_every=haar_every))
+            for n, d, s in zip(ns, ds, seeds)]
+
+
+def make_batch(circuits, precision="c128", k_by_n=None, wg_max_qubits=DEFAULT_WG_MAX_QUBITS,
+               device=0, build=True):
+    """A ``Batch`` holding ``circuits`` ([(n, gates), ...]), planned and built."""
+    b = Batch(precision, device)
+    for n, gates in circuits:
+        b.add_circuit(n, gates)
+    b.set_plan(dict(k_by_n or {}), int(wg_max_qubits))
+    if build:
+        b.build()
+    return b
+
+
+def _time_batch(b, reps):
+    b.run()  # warm
+    return min(b.run() for _ in range(reps))
+
+
+def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 << 24,
+             reps=3, seed=7, device=0, wg_limit=20, layer_floor=12, log=None):
+    """Pick the fusion width per qubit count and the workgroup/layered split.
+
+    For every n in ``ns`` a probe batch of copies of one depth-``depth``
+    random circuit (enough copies to hold about ``amp_budget`` amplitudes,
+    8..2048) is timed for every width in ``widths``, on the persistent workgroup
+    path (n <= ``wg_limit``) and on the layered path (n >= ``layer_floor``).
+    Returns ``{"k_by_n": {n: k}, "wg_max_qubits": t, "probe_ms": {...}}``
+    where t is the largest n whose best workgroup time beats its best
+    layered time (the split is monotone in n by construction).
+    """
+    ns = sorted(set(int(n) for n in ns))
+    best = {}
+    probe = {}
+    for n in ns:
+        copies = max(8, min(2048, amp_budget >> n))
+        b = Batch(precision, device)
+        b.add_circuit(n, random_layers_circuit(n, depth, seed + n, haar_every=5))
+        b.add_copies(0, copies - 1)
+        for path, limit in (("wg", 24), ("layer", 0)):
+            if path == "wg" and n > wg_limit:
+                continue
+            if path == "layer" and n < layer_floor:
+                continue
+            for k in widths:
+                if k > n:
+                    continue
+                b.set_plan({n: k}, limit)
+                b.build()
+                ms = _time_batch(b, reps)
+                probe[(n, path, k)] = ms
+                if log:
+                    log("autotune n=%d %s k=%d: %.3f ms (%d circuits)" % (n, path, k, ms, copies))
+                if (n, path) not in best or ms < best[(n, path)][1]:
+                    best[(n, path)] = (k, ms)
+        del b
+    wg_max = 0
+    for n in ns:
+        wg = best.get((n, "wg"))
+        ly = best.get((n, "layer"))
+        if wg and (ly is None or wg[1] <= ly[1]):
+            wg_max = n
+        else:
+            break
+    k_by_n = {}
+    for n in ns:
+        path = "wg" if n <= wg_max else "layer"
+        k_by_n[n] = best[(n, path)][0] if (n, path) in best else 4
+    return {"k_by_n": k_by_n, "wg_max_qubits": wg_max,
+            "probe_ms": {"%d/%s/k%d" % key: round(v, 4) for key, v in probe.items()}}
+
+
+def one_at_a_time(circuits, precision="c128", k=4, device=0):
+    """The plain path: one SimulatorMPI per circuit, gates fed through
+    apply_controlled_gate with run() per fused cluster of at most k qubits
+    (lookahead plan), the final state synchronised.  Returns seconds."""
+    from . import SimulatorMPI
+    from .scheduling import plan_clusters
+    plans = [plan_clusters(g, list(range(n)), k, mode="lookahead") for n, g in circuits]
+    t0 = time.perf_counter()
+    for (n, gates), clusters in zip(circuits, plans):
+        sim = SimulatorMPI(1, n, k, precision, device)
+        sim.allocate_qureg(list(range(n)), 0)
+        for cl in clusters:
+            for i in cl:
+                m, t, c = gates[i]
+                sim.apply_controlled_gate(m, t, c)
+            sim.run()
+        sim.synchronize()
+        del sim
+    return time.perf_counter() - t0
+
+
+def fidelity(a, b):
+    """|<a|b>|^2 / (|a|^2 |b|^2)."""
+    return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+def workload_bytes(circuits, precision):
+    """State bytes of a list of circuits."""
+    ab = 16 if precision == "c128" else 8
+    return sum(ab << n for n, _ in circuits)

@@ -62,12 +62,19 @@ def random_layers_circuit(n_qubits, n_layers, seed, haar_every=0):
     for layer in range(n_layers):
         kinds = rng.integers(0, 4, size=n_qubits)
         thetas = rng.uniform(0.0, 2.0 * math.pi, size=n_qubits)
-        for q in range(n_qubits):
-            k = int(kinds[q])
-            m = H if k == 0 else (Rx, Ry, Rz)[k - 1](float(thetas[q]))
-            gates.append((m, [q], []))
-        for i in range(layer % 2, n_qubits - 1, 2):
-            gates.append((X, [i + 1], [i]))
+        c, s = np.cos(thetas / 2), np.sin(thetas / 2)
+        m = np.zeros((n_qubits, 2, 2), dtype=np.complex128)
+        m[kinds == 0] = H
+        rx, ry, rz = kinds == 1, kinds == 2, kinds == 3
+        m[rx, 0, 0] = m[rx, 1, 1] = c[rx]
+        m[rx, 0, 1] = m[rx, 1, 0] = -1j * s[rx]
+        m[ry, 0, 0] = m[ry, 1, 1] = c[ry]
+        m[ry, 0, 1] = -s[ry]
+        m[ry, 1, 0] = s[ry]
+        m[rz, 0, 0] = np.exp(-0.5j * thetas[rz])
+        m[rz, 1, 1] = np.exp(0.5j * thetas[rz])
+        gates.extend(zip(list(m), ([q] for q in range(n_qubits)), ([] for _ in range(n_qubits))))
+        gates.extend((X, [i + 1], [i]) for i in range(layer % 2, n_qubits - 1, 2))
         if haar_every and (layer + 1) % haar_every == 0 and n_qubits >= 2:
             a, b = rng.choice(n_qubits, size=2, replace=False)
             gates.append((haar_unitary(4, rng), [int(a), int(b)], []))

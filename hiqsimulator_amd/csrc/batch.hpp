@@ -1,0 +1,191 @@
+// Many independent circuits in one device arena.
+//
+// The reference runs one circuit per simulator object: gates are queued
+// (SimulatorMPI::ApplyGate), fused and applied on run() (SimulatorMPI::Run,
+// SimulatorMPI.cpp:407-436).  A batch holds many such circuits, each from
+// |0...0> on its own qubits 0..n-1 (bit p of a state's index = qubit p), and
+// applies all of them in one graph launch:
+//  * every circuit is cut into fused clusters (plan_clusters, lookahead
+//    packer, at most k qubits per cluster with k chosen per qubit count) and
+//    each cluster becomes one dense 2^k x 2^k pass (controls folded into the
+//    matrix as diag(I, U) on the control bits);
+//  * small states are owned by one workgroup each inside one persistent
+//    launch: states of up to 128 KiB live in LDS for the whole circuit and
+//    are written to HBM once; larger ones up to a size limit are streamed by
+//    their workgroup pass after pass;
+//  * large states are advanced layer by layer: launch L applies the L-th
+//    pass of every large circuit (one launch per fused width), many
+//    workgroups per state;
+//  * the whole sequence is captured once into a hipGraph and replayed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace hq {
+
+// ---- device descriptors ---------------------------------------------------
+// One fused pass of a workgroup-owned circuit.
+struct BatchPass {
+     uint64_t mat_off;  // first complex entry of the 2^k x 2^k matrix (row-major)
+     uint8_t k;
+     uint8_t pos[5];  // ascending bit positions; matrix bit l <-> pos[l]
+     uint8_t pad[2];
+};
+
+// A workgroup-owned circuit.
+struct BatchCirc {
+     uint64_t state_off;  // first amplitude of the state in the arena
+     uint32_t pass_begin, pass_end;
+     uint32_t n;
+     uint32_t pad;
+};
+
+// One pass of one large circuit inside a layered launch; items of a launch
+// share k and lie back to back in blocks of kBatchLayerBlock groups.
+struct BatchItem {
+     uint64_t state_off;
+     uint64_t mat_off;
+     uint64_t block_start;  // first block of the item within the launch
+     uint32_t log_groups;   // log2 of the 2^(n-k) groups
+     uint8_t pos[5];        // ascending
+     uint8_t k;             // fused width (launches that mix widths read it)
+     uint8_t pad[2];
+};
+
+constexpr int kBatchLayerBlock = 256;
+// A state of at most this many bytes stays in LDS for its whole circuit.
+constexpr size_t kBatchLdsBytes = 128 * 1024;
+
+// Persistent launch over the workgroup-owned circuits: workgroup w runs
+// circuits wg_list[wg_off[w] .. wg_off[w+1]) from |0..0> to the end.
+// kmax is the widest pass (<= 5); with the precision it picks the
+// workgroup size (batch_wg_threads).
+hipError_t launch_batch_wg(void* arena, Prec prec, const void* mats, const BatchPass* passes,
+                           const BatchCirc* circs, const uint32_t* wg_off,
+                           const uint32_t* wg_list, int n_wg, int kmax, hipStream_t s);
+int batch_wg_threads(Prec prec, int kmax);
+
+// One layered launch: every item's pass, total_blocks = sum of item blocks.
+hipError_t launch_batch_layer(void* arena, Prec prec, const void* mats, const BatchItem* items,
+                              int nitems, int k, uint64_t total_blocks, hipStream_t s);
+
+// arena[offsets[c]] = 1 for c < ncirc.
+hipError_t launch_set_ones(void* arena, Prec prec, const uint64_t* offsets, int ncirc,
+                           hipStream_t s);
+
+// ---- serving: launches over requests at different points of their circuits
+// One pass each of many states, widths mixed (item.k <= kmax).
+hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const BatchItem* items,
+                              int nitems, uint64_t total_blocks, int kmax, hipStream_t s);
+// Amplitude blocks per item of launch_init_states.
+constexpr int kInitAmpsPerBlock = 1024;
+// |0..0> for each item's state: item {state_off, block_start, log_groups = n}.
+hipError_t launch_init_states(void* arena, Prec prec, const BatchItem* items, int nitems,
+                              uint64_t total_blocks, hipStream_t s);
+// Probability of each block of 2^log_groups amplitudes: item {state_off,
+// mat_off = first output slot, block_start, log_groups}; one workgroup per
+// block; out[mat_off + j] = sum over block j of |psi|^2 (FP64).
+hipError_t launch_block_sums_multi(const void* arena, Prec prec, const BatchItem* items,
+                                   int nitems, uint64_t total_blocks, double* out, hipStream_t s);
+
+// Sampling job: `shots` computational-basis outcomes of one state, shot s
+// drawing u_s from Philox4x32-10 (counter s, key = seed).  sums[sums_off ..
+// + 2^log_blocks) holds the block probabilities (launch_block_sums_multi);
+// they are turned into a running sum in place.
+struct SampleJob {
+     uint64_t state_off;
+     uint64_t sums_off;
+     uint64_t out_off;
+     uint64_t seed;
+     uint32_t log_blocks;
+     uint32_t log_block_amps;
+     uint32_t shots;
+     uint32_t pad;
+};
+// Probability blocks of a sample job over 2^n amplitudes: blocks of at
+// least 256 amplitudes, at most 2^15 blocks.
+inline uint32_t sample_log_blocks(int n)
+{
+     const int lb = n - 8;
+     return static_cast<uint32_t>(lb < 0 ? 0 : (lb > 15 ? 15 : lb));
+}
+// One workgroup per job; out[out_off + s] = outcome of shot s.
+hipError_t launch_sample(const void* arena, Prec prec, const SampleJob* jobs, int njobs,
+                         double* sums, uint64_t* out, hipStream_t s);
+
+// ---- host side ------------------------------------------------------------
+struct BatchGate {
+     std::vector<std::complex<double>> m;  // 2^t x 2^t, row-major, bit l <-> targets[l]
+     std::vector<int64_t> targets;
+     std::vector<int64_t> ctrls;
+};
+
+struct BatchStats {
+     uint64_t circuits = 0;
+     uint64_t passes = 0;           // fused passes over all circuits
+     uint64_t wg_circuits = 0;      // circuits owned by one workgroup
+     uint64_t lds_circuits = 0;     // ... of which LDS-resident
+     uint64_t layer_launches = 0;   // layered launches in the graph
+     uint64_t arena_bytes = 0;
+     uint64_t matrix_bytes = 0;
+     uint64_t amp_passes = 0;       // sum over passes of 2^n (amplitudes touched)
+     int wg_grid = 0;
+     int kmax = 0;                  // widest fused pass
+};
+
+class Batch {
+public:
+     Batch(Prec prec, int device);
+     ~Batch();
+     Batch(const Batch&) = delete;
+     Batch& operator=(const Batch&) = delete;
+
+     // Returns the circuit's index.  Qubits are 0..n-1; every gate id must be
+     // below n and a gate's targets and controls distinct.
+     int64_t add_circuit(int n, std::vector<BatchGate> gates);
+     // Appends `count` copies of circuit c; returns the first copy's index.
+     int64_t add_copies(int64_t c, int64_t count);
+     // k_by_n[n] = fusion width for circuits of n qubits (0 = default 4);
+     // circuits of more than wg_max_qubits qubits take the layered path.
+     void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits);
+     // Plans, fuses, uploads and captures the graph (replaces a previous build).
+     void build();
+     // Replays the graph and waits; returns the device time in ms.
+     double run();
+     void get_state(int64_t c, std::complex<double>* out) const;
+     int num_qubits(int64_t c) const;
+     int64_t num_circuits() const { return static_cast<int64_t>(circ_.size()); }
+     const BatchStats& stats() const { return stats_; }
+     int fusion_width(int n) const;
+
+private:
+     struct Circ {
+          int n;
+          std::vector<BatchGate> gates;
+          uint64_t state_off = 0;
+     };
+     void release_device();
+
+     Prec prec_;
+     int device_;
+     hipStream_t s1_ = nullptr, s2_ = nullptr;
+     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr, ev_t0_ = nullptr, ev_t1_ = nullptr;
+     hipGraph_t graph_ = nullptr;
+     hipGraphExec_t exec_ = nullptr;
+     std::vector<Circ> circ_;
+     std::vector<int> k_by_n_;
+     int wg_max_qubits_ = 16;
+     bool built_ = false;
+     void* d_arena_ = nullptr;
+     void* d_mats_ = nullptr;
+     void* d_desc_ = nullptr;  // passes, circs, wg lists, items, one-offsets
+     BatchStats stats_;
+};
+
+}  // namespace hq

@@ -1,0 +1,493 @@
+// gfx950 kernels of the batched many-circuit path (batch.hpp).
+//
+// Each fused pass is the dense k-qubit update of the single-state engine
+// (kernels/intrin/kernel{1..5}.hpp of the reference: for every base I with
+// the target bits 0, v_j = psi[I + off_j], psi[I + off_i] = sum_j m[i][j] v_j),
+// applied to many states:
+//  * k_batch_wg: persistent; each workgroup owns whole circuits.  A state of
+//    at most 128 KiB is initialised to |0..0> in LDS, every pass of the
+//    circuit runs on LDS with a workgroup barrier between passes, and the
+//    final state is written to HBM once.  A larger state is streamed through
+//    L2/HBM by its workgroup in the same way.  No inter-workgroup
+//    synchronisation exists, so every workgroup ends after its own list.
+//  * k_batch_layer: one pass each of many large circuits; blocks map to
+//    items by binary search over the items' first blocks.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "batch.hpp"
+#include "device_common.hpp"
+
+namespace hq {
+namespace {
+
+template <typename V>
+__device__ __forceinline__ V basis_amp(bool one)
+{
+     V a;
+     a.x = one ? 1 : 0;
+     a.y = 0;
+     return a;
+}
+
+// One fused pass of one workgroup over a state of 2^n amplitudes at psi
+// (LDS or global memory); mat is the pass's matrix in LDS, pos ascending.
+// The rows are not unrolled: a row is read (uniform address, broadcast),
+// reduced against the group held in registers and stored before the next.
+template <int K, typename R, int NT>
+__device__ __forceinline__ void wg_pass(VT<R>* psi, uint32_t n, const VT<R>* mat,
+                                        const uint8_t (&pos)[5])
+{
+     using V = VT<R>;
+     constexpr int D = 1 << K;
+     const uint32_t ng = 1u << (n - K);
+     uint32_t off[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          uint32_t o = 0;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) o |= 1u << pos[l];
+          off[j] = o;
+     }
+     for (uint32_t t = threadIdx.x; t < ng; t += NT) {
+          uint32_t b = t;
+#pragma unroll
+          for (int l = 0; l < K; ++l) {
+               const uint32_t lo = b & ((1u << pos[l]) - 1u);
+               b = ((b ^ lo) << 1) | lo;
+          }
+          V v[D];
+#pragma unroll
+          for (int j = 0; j < D; ++j) v[j] = psi[b + off[j]];
+#pragma unroll 1
+          for (int i = 0; i < D; ++i) {
+               const V* const row = mat + i * D;
+               R re = 0, im = 0;
+#pragma unroll
+               for (int j = 0; j < D; ++j) {
+                    const V m = row[j];
+                    re = fma(m.x, v[j].x, re);
+                    re = fma(-m.y, v[j].y, re);
+                    im = fma(m.x, v[j].y, im);
+                    im = fma(m.y, v[j].x, im);
+               }
+               uint32_t oi = 0;
+#pragma unroll
+               for (int l = 0; l < K; ++l)
+                    if ((i >> l) & 1) oi |= 1u << pos[l];
+               V o;
+               o.x = re;
+               o.y = im;
+               psi[b + oi] = o;
+          }
+     }
+}
+
+// |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
+// staged in LDS (smat) and read from there with uniform-address loads, which
+// keeps the scalar registers free for the group offsets.
+template <typename R, int KMAX, int NT>
+__device__ __forceinline__ void wg_circuit(VT<R>* psi, VT<R>* smat, const BatchCirc& c,
+                                           const BatchPass* __restrict__ passes,
+                                           const VT<R>* __restrict__ mats)
+{
+     using V = VT<R>;
+     const uint32_t N = 1u << c.n;
+     for (uint32_t i = threadIdx.x; i < N; i += NT) psi[i] = basis_amp<V>(i == 0);
+     for (uint32_t p = c.pass_begin; p < c.pass_end; ++p) {
+          const BatchPass ps = passes[p];
+          const uint32_t dd = 1u << (2 * ps.k);
+          for (uint32_t i = threadIdx.x; i < dd; i += NT) smat[i] = mats[ps.mat_off + i];
+          __syncthreads();
+          switch (ps.k) {
+               case 1: wg_pass<1, R, NT>(psi, c.n, smat, ps.pos); break;
+               case 2: wg_pass<2, R, NT>(psi, c.n, smat, ps.pos); break;
+               case 3: wg_pass<3, R, NT>(psi, c.n, smat, ps.pos); break;
+               case 4: wg_pass<4, R, NT>(psi, c.n, smat, ps.pos); break;
+               default:
+                    if constexpr (KMAX >= 5) wg_pass<5, R, NT>(psi, c.n, smat, ps.pos);
+                    break;
+          }
+          __syncthreads();
+     }
+}
+
+template <typename R, int KMAX, int NT>
+__global__ void __launch_bounds__(NT)
+    k_batch_wg(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+               const BatchPass* __restrict__ passes, const BatchCirc* __restrict__ circs,
+               const uint32_t* __restrict__ wg_off, const uint32_t* __restrict__ wg_list)
+{
+     using V = VT<R>;
+     constexpr uint32_t kLds = kBatchLdsBytes / sizeof(V);
+     __shared__ V lds[kLds];
+     __shared__ V smat[1 << (2 * KMAX)];
+     const uint32_t c0 = wg_off[blockIdx.x], c1 = wg_off[blockIdx.x + 1];
+     for (uint32_t ci = c0; ci < c1; ++ci) {
+          const BatchCirc c = circs[wg_list[ci]];
+          const uint32_t N = 1u << c.n;
+          V* const g = arena + c.state_off;
+          if (N <= kLds) {
+               wg_circuit<R, KMAX, NT>(lds, smat, c, passes, mats);
+               for (uint32_t i = threadIdx.x; i < N; i += NT) st_nt(g + i, lds[i]);
+               __syncthreads();
+          }
+          else {
+               wg_circuit<R, KMAX, NT>(g, smat, c, passes, mats);
+          }
+     }
+}
+
+template <int K, typename R>
+__global__ void __launch_bounds__(kBatchLayerBlock)
+    k_batch_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+                  const BatchItem* __restrict__ items, const int nitems)
+{
+     using V = VT<R>;
+     constexpr int D = 1 << K;
+     const uint64_t b = block_linear();
+     int lo = 0, hi = nitems - 1;
+     while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (items[mid].block_start <= b) lo = mid;
+          else hi = mid - 1;
+     }
+     const BatchItem it = items[lo];
+     const uint64_t t = (b - it.block_start) * kBatchLayerBlock + threadIdx.x;
+     if ((t >> it.log_groups) != 0) return;
+     uint64_t base = t;
+     uint64_t off[D];
+#pragma unroll
+     for (int l = 0; l < K; ++l) {
+          const uint64_t lo_bits = base & ((1ull << it.pos[l]) - 1ull);
+          base = ((base ^ lo_bits) << 1) | lo_bits;
+     }
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          uint64_t o = 0;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) o |= 1ull << it.pos[l];
+          off[j] = o;
+     }
+     V* const p = arena + it.state_off + base;
+     V v[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) v[j] = p[off[j]];
+     group_matvec<K, V, R>(mats + it.mat_off, v, [&](int i, V o) { p[off[i]] = o; });
+}
+
+// Item lookup of a block-mapped launch: the last item whose first block is
+// at or below b.
+__device__ __forceinline__ int find_item(const BatchItem* __restrict__ items, int nitems, uint64_t b)
+{
+     int lo = 0, hi = nitems - 1;
+     while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (items[mid].block_start <= b) lo = mid;
+          else hi = mid - 1;
+     }
+     return lo;
+}
+
+template <int K, typename R>
+__device__ __forceinline__ void layer_group(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+                                            const BatchItem& it, uint64_t t)
+{
+     using V = VT<R>;
+     constexpr int D = 1 << K;
+     uint64_t base = t;
+     uint64_t off[D];
+#pragma unroll
+     for (int l = 0; l < K; ++l) {
+          const uint64_t lo_bits = base & ((1ull << it.pos[l]) - 1ull);
+          base = ((base ^ lo_bits) << 1) | lo_bits;
+     }
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          uint64_t o = 0;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) o |= 1ull << it.pos[l];
+          off[j] = o;
+     }
+     V* const p = arena + it.state_off + base;
+     V v[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) v[j] = p[off[j]];
+     group_matvec<K, V, R>(mats + it.mat_off, v, [&](int i, V o) { p[off[i]] = o; });
+}
+
+template <typename R, int KMAX>
+__global__ void __launch_bounds__(kBatchLayerBlock)
+    k_serve_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+                  const BatchItem* __restrict__ items, const int nitems)
+{
+     const uint64_t b = block_linear();
+     const BatchItem it = items[find_item(items, nitems, b)];
+     const uint64_t t = (b - it.block_start) * kBatchLayerBlock + threadIdx.x;
+     if ((t >> it.log_groups) != 0) return;
+     switch (it.k) {
+          case 1: layer_group<1, R>(arena, mats, it, t); break;
+          case 2: layer_group<2, R>(arena, mats, it, t); break;
+          case 3: layer_group<3, R>(arena, mats, it, t); break;
+          case 4: layer_group<4, R>(arena, mats, it, t); break;
+          default:
+               if constexpr (KMAX >= 5) layer_group<5, R>(arena, mats, it, t);
+               break;
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBatchLayerBlock)
+    k_init_states(VT<R>* __restrict__ arena, const BatchItem* __restrict__ items, const int nitems)
+{
+     using V = VT<R>;
+     constexpr int E = kInitAmpsPerBlock / kBatchLayerBlock;
+     const uint64_t b = block_linear();
+     const BatchItem it = items[find_item(items, nitems, b)];
+     const uint64_t first = (b - it.block_start) * kInitAmpsPerBlock;
+     const uint64_t N = 1ull << it.log_groups;
+#pragma unroll
+     for (int e = 0; e < E; ++e) {
+          const uint64_t i = first + static_cast<uint64_t>(e) * kBatchLayerBlock + threadIdx.x;
+          if (i < N) arena[it.state_off + i] = basis_amp<V>(i == 0);
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBatchLayerBlock)
+    k_block_sums_multi(const VT<R>* __restrict__ arena, const BatchItem* __restrict__ items,
+                       const int nitems, double* __restrict__ out)
+{
+     const uint64_t b = block_linear();
+     const BatchItem it = items[find_item(items, nitems, b)];
+     const uint64_t j = b - it.block_start;
+     const uint64_t K = 1ull << it.log_groups;
+     const VT<R>* const p = arena + it.state_off + j * K;
+     double acc = 0.0;
+     for (uint64_t i = threadIdx.x; i < K; i += kBatchLayerBlock) {
+          const double re = p[i].x, im = p[i].y;
+          acc += re * re + im * im;
+     }
+     __shared__ double red[kBatchLayerBlock / 64];
+     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+     __syncthreads();
+     if (threadIdx.x == 0) {
+          double r = 0;
+          for (int w = 0; w < kBatchLayerBlock / 64; ++w) r += red[w];
+          out[it.mat_off + j] = r;
+     }
+}
+
+constexpr int kSampleThreads = 1024;
+
+// One workgroup per job: running sum of the block probabilities (in place),
+// then per shot u = U[0,1) * total, the first block whose running sum
+// exceeds u, and a scan of that block's amplitudes.
+template <typename R>
+__global__ void __launch_bounds__(kSampleThreads)
+    k_sample(const VT<R>* __restrict__ arena, const SampleJob* __restrict__ jobs,
+             double* __restrict__ sums, uint64_t* __restrict__ out)
+{
+     const SampleJob jb = jobs[blockIdx.x];
+     double* const cdf = sums + jb.sums_off;
+     const uint32_t nb = 1u << jb.log_blocks;
+     const uint32_t per = nb > kSampleThreads ? nb / kSampleThreads : 1u;
+     const uint32_t t = threadIdx.x;
+     const uint32_t first = t * per;
+     double local = 0.0;
+     if (first < nb)
+          for (uint32_t i = 0; i < per; ++i) local += cdf[first + i];
+     __shared__ double part[kSampleThreads];
+     part[t] = local;
+     __syncthreads();
+     for (uint32_t o = 1; o < kSampleThreads; o <<= 1) {
+          const double add = t >= o ? part[t - o] : 0.0;
+          __syncthreads();
+          part[t] += add;
+          __syncthreads();
+     }
+     const double total = part[kSampleThreads - 1];
+     if (first < nb) {
+          double run = t ? part[t - 1] : 0.0;
+          for (uint32_t i = 0; i < per; ++i) {
+               run += cdf[first + i];
+               cdf[first + i] = run;
+          }
+     }
+     __syncthreads();
+     const uint64_t K = 1ull << jb.log_block_amps;
+     const VT<R>* const psi = arena + jb.state_off;
+     const uint2 key = make_uint2(static_cast<uint32_t>(jb.seed), static_cast<uint32_t>(jb.seed >> 32));
+     for (uint32_t s = t; s < jb.shots; s += kSampleThreads) {
+          const uint4 r = philox4x32_10(make_uint4(s, 0u, 0x5a3du, 0u), key);
+          const double u = uniform53(r.x, r.y) * total;
+          uint32_t lo = 0, hi = nb - 1;  // first block with cdf > u
+          while (lo < hi) {
+               const uint32_t mid = (lo + hi) >> 1;
+               if (cdf[mid] > u) hi = mid;
+               else lo = mid + 1;
+          }
+          double rem = u - (lo ? cdf[lo - 1] : 0.0);
+          const uint64_t b0 = static_cast<uint64_t>(lo) * K;
+          uint64_t pick = b0, last_nz = b0;
+          bool found = false;
+          for (uint64_t i = 0; i < K; ++i) {
+               const double re = psi[b0 + i].x, im = psi[b0 + i].y;
+               const double pr = re * re + im * im;
+               if (pr > 0.0) last_nz = b0 + i;
+               if (rem < pr) {
+                    pick = b0 + i;
+                    found = true;
+                    break;
+               }
+               rem -= pr;
+          }
+          out[jb.out_off + s] = found ? pick : last_nz;
+     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock)
+    k_set_ones(VT<R>* __restrict__ arena, const uint64_t* __restrict__ offsets, const int ncirc)
+{
+     const int c = blockIdx.x * kBlock + threadIdx.x;
+     if (c >= ncirc) return;
+     arena[offsets[c]] = basis_amp<VT<R>>(true);
+}
+
+}  // namespace
+
+int batch_wg_threads(Prec prec, int kmax) { return prec == Prec::C64 && kmax <= 4 ? 1024 : 512; }
+
+hipError_t launch_batch_wg(void* arena, Prec prec, const void* mats, const BatchPass* passes,
+                           const BatchCirc* circs, const uint32_t* wg_off,
+                           const uint32_t* wg_list, int n_wg, int kmax, hipStream_t s)
+{
+     if (n_wg <= 0) return hipSuccess;
+     if (kmax < 1 || kmax > 5) return hipErrorInvalidValue;
+#define HQ_WG(RR, KM, NT)                                                                        \
+     hipLaunchKernelGGL((k_batch_wg<RR, KM, NT>), dim3(n_wg), dim3(NT), 0, s,                     \
+                        static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), passes,    \
+                        circs, wg_off, wg_list)
+     if (prec == Prec::C128) {
+          if (kmax >= 5) HQ_WG(double, 5, 512);
+          else HQ_WG(double, 4, 512);
+     }
+     else {
+          if (kmax >= 5) HQ_WG(float, 5, 512);
+          else HQ_WG(float, 4, 1024);
+     }
+#undef HQ_WG
+     return hipGetLastError();
+}
+
+hipError_t launch_batch_layer(void* arena, Prec prec, const void* mats, const BatchItem* items,
+                              int nitems, int k, uint64_t total_blocks, hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     const dim3 grid = grid_2d(total_blocks);
+#define HQ_LAYER(KK)                                                                             \
+     do {                                                                                         \
+          if (prec == Prec::C128)                                                                 \
+               hipLaunchKernelGGL((k_batch_layer<KK, double>), grid, dim3(kBatchLayerBlock), 0,   \
+                                  s, static_cast<double2*>(arena),                                \
+                                  static_cast<const double2*>(mats), items, nitems);              \
+          else                                                                                    \
+               hipLaunchKernelGGL((k_batch_layer<KK, float>), grid, dim3(kBatchLayerBlock), 0, s, \
+                                  static_cast<float2*>(arena), static_cast<const float2*>(mats),  \
+                                  items, nitems);                                                 \
+     } while (0)
+     switch (k) {
+          case 1: HQ_LAYER(1); break;
+          case 2: HQ_LAYER(2); break;
+          case 3: HQ_LAYER(3); break;
+          case 4: HQ_LAYER(4); break;
+          case 5: HQ_LAYER(5); break;
+          default: return hipErrorInvalidValue;
+     }
+#undef HQ_LAYER
+     return hipGetLastError();
+}
+
+hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const BatchItem* items,
+                              int nitems, uint64_t total_blocks, int kmax, hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     if (kmax < 1 || kmax > 5) return hipErrorInvalidValue;
+     const dim3 grid = grid_2d(total_blocks);
+#define HQ_SERVE(RR, KM)                                                                         \
+     hipLaunchKernelGGL((k_serve_layer<RR, KM>), grid, dim3(kBatchLayerBlock), 0, s,              \
+                        static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
+                        nitems)
+     if (prec == Prec::C128) {
+          if (kmax >= 5) HQ_SERVE(double, 5);
+          else HQ_SERVE(double, 4);
+     }
+     else {
+          if (kmax >= 5) HQ_SERVE(float, 5);
+          else HQ_SERVE(float, 4);
+     }
+#undef HQ_SERVE
+     return hipGetLastError();
+}
+
+hipError_t launch_init_states(void* arena, Prec prec, const BatchItem* items, int nitems,
+                              uint64_t total_blocks, hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     const dim3 grid = grid_2d(total_blocks);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_init_states<double>), grid, dim3(kBatchLayerBlock), 0, s,
+                             static_cast<double2*>(arena), items, nitems);
+     else
+          hipLaunchKernelGGL((k_init_states<float>), grid, dim3(kBatchLayerBlock), 0, s,
+                             static_cast<float2*>(arena), items, nitems);
+     return hipGetLastError();
+}
+
+hipError_t launch_block_sums_multi(const void* arena, Prec prec, const BatchItem* items,
+                                   int nitems, uint64_t total_blocks, double* out, hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     const dim3 grid = grid_2d(total_blocks);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_block_sums_multi<double>), grid, dim3(kBatchLayerBlock), 0, s,
+                             static_cast<const double2*>(arena), items, nitems, out);
+     else
+          hipLaunchKernelGGL((k_block_sums_multi<float>), grid, dim3(kBatchLayerBlock), 0, s,
+                             static_cast<const float2*>(arena), items, nitems, out);
+     return hipGetLastError();
+}
+
+hipError_t launch_sample(const void* arena, Prec prec, const SampleJob* jobs, int njobs,
+                         double* sums, uint64_t* out, hipStream_t s)
+{
+     if (njobs <= 0) return hipSuccess;
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_sample<double>), dim3(njobs), dim3(kSampleThreads), 0, s,
+                             static_cast<const double2*>(arena), jobs, sums, out);
+     else
+          hipLaunchKernelGGL((k_sample<float>), dim3(njobs), dim3(kSampleThreads), 0, s,
+                             static_cast<const float2*>(arena), jobs, sums, out);
+     return hipGetLastError();
+}
+
+hipError_t launch_set_ones(void* arena, Prec prec, const uint64_t* offsets, int ncirc, hipStream_t s)
+{
+     if (ncirc <= 0) return hipSuccess;
+     const unsigned grid = static_cast<unsigned>((ncirc + kBlock - 1) / kBlock);
+     if (prec == Prec::C128)
+          hipLaunchKernelGGL((k_set_ones<double>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<double2*>(arena), offsets, ncirc);
+     else
+          hipLaunchKernelGGL((k_set_ones<float>), dim3(grid), dim3(kBlock), 0, s,
+                             static_cast<float2*>(arena), offsets, ncirc);
+     return hipGetLastError();
+}
+
+}  // namespace hq

@@ -5,13 +5,27 @@
 #include <vector>
 
 #include "../../include/hiqsim_amd.h"
+#include "batch.hpp"
 #include "engine.hpp"
 #include "sched.hpp"
+#include "serve.hpp"
 
 struct hiqsim_engine {
      hq::Engine eng;
      hiqsim_engine(uint64_t seed, int64_t ml, int64_t mc, hq::Prec p, int dev)
          : eng(seed, ml, mc, p, dev)
+     {}
+};
+
+struct hiqsim_batch {
+     hq::Batch b;
+     hiqsim_batch(hq::Prec p, int dev) : b(p, dev) {}
+};
+
+struct hiqsim_server {
+     hq::Server s;
+     hiqsim_server(hq::Prec p, int dev, uint64_t sb, uint64_t mb, int pps, int ma)
+         : s(p, dev, sb, mb, pps, ma)
      {}
 };
 
@@ -73,6 +87,36 @@ void pauli_args(const uint64_t* x, const uint64_t* z, const double* c, size_t nt
      Z.assign(z, z + nt);
      C.resize(nt);
      for (size_t t = 0; t < nt; ++t) C[t] = hq::cplx(c[2 * t], c[2 * t + 1]);
+}
+
+// Gate lists of the batch and request-loop entry points (CSR targets and
+// controls, matrices back to back).
+std::vector<hq::BatchGate> batch_gates(size_t ngates, const double* mats, const int64_t* tgt_ids,
+                                       const size_t* tgt_off, const int64_t* ctrl_ids,
+                                       const size_t* ctrl_off)
+{
+     std::vector<hq::BatchGate> gates(ngates);
+     if (ngates) {
+          need(mats, "mats");
+          need(tgt_ids, "tgt_ids");
+          need(tgt_off, "tgt_off");
+          need(ctrl_off, "ctrl_off");
+          if (ctrl_off[ngates] > ctrl_off[0]) need(ctrl_ids, "ctrl_ids");
+     }
+     size_t m = 0;
+     for (size_t g = 0; g < ngates; ++g) {
+          if (tgt_off[g + 1] < tgt_off[g] || ctrl_off[g + 1] < ctrl_off[g])
+               throw std::runtime_error("gate offsets must be non-decreasing");
+          const size_t nt = tgt_off[g + 1] - tgt_off[g];
+          if (nt == 0 || nt > 5) throw std::runtime_error("a gate needs 1..5 targets");
+          gates[g].targets.assign(tgt_ids + tgt_off[g], tgt_ids + tgt_off[g + 1]);
+          gates[g].ctrls.assign(ctrl_ids + ctrl_off[g], ctrl_ids + ctrl_off[g + 1]);
+          const size_t dd = (size_t(1) << nt) * (size_t(1) << nt);
+          gates[g].m.resize(dd);
+          for (size_t i = 0; i < dd; ++i) gates[g].m[i] = hq::cplx(mats[2 * (m + i)], mats[2 * (m + i) + 1]);
+          m += dd;
+     }
+     return gates;
 }
 
 hq::GateList gate_list(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
@@ -424,6 +468,263 @@ int hiqsim_schedule_swap(const int64_t* gate_ids, const size_t* gate_off, const 
           if (!r.empty()) need(out, "out");
           for (size_t i = 0; i < r.size(); ++i) out[i] = r[i];
           *nout = r.size();
+     });
+}
+
+int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, size_t ngates, int cluster_size, int mode,
+                         int32_t* cluster_of)
+{
+     return guard([&] {
+          if (ngates) need(cluster_of, "cluster_of");
+          std::vector<uint8_t> diag(ngates, 0);
+          auto g = gate_list(gate_ids, gate_off, ctrl_ids, ctrl_off, diag.data(), ngates);
+          auto r = hq::plan_clusters(g, cluster_size, mode);
+          for (size_t i = 0; i < r.size(); ++i) cluster_of[i] = r[i];
+     });
+}
+
+int hiqsim_batch_create(int precision, int device, hiqsim_batch_t** out)
+{
+     return guard([&] {
+          need(out, "out");
+          if (precision != HIQSIM_C128 && precision != HIQSIM_C64)
+               throw std::runtime_error("precision must be HIQSIM_C128 or HIQSIM_C64");
+          *out = new hiqsim_batch(static_cast<hq::Prec>(precision), device);
+     });
+}
+
+int hiqsim_batch_destroy(hiqsim_batch_t* b)
+{
+     return guard([&] { delete b; });
+}
+
+int hiqsim_batch_add_circuit(hiqsim_batch_t* b, int n_qubits, size_t ngates, const double* mats,
+                             const int64_t* tgt_ids, const size_t* tgt_off,
+                             const int64_t* ctrl_ids, const size_t* ctrl_off, int64_t* index)
+{
+     return guard([&] {
+          need(b, "batch");
+          need(index, "index");
+          *index = b->b.add_circuit(n_qubits, batch_gates(ngates, mats, tgt_ids, tgt_off, ctrl_ids,
+                                                          ctrl_off));
+     });
+}
+
+int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int64_t* first)
+{
+     return guard([&] {
+          need(b, "batch");
+          const int64_t f = b->b.add_copies(index, count);
+          if (first) *first = f;
+     });
+}
+
+int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits)
+{
+     return guard([&] {
+          need(b, "batch");
+          if (len) need(k_by_n, "k_by_n");
+          b->b.set_plan(std::vector<int>(k_by_n, k_by_n + len), wg_max_qubits);
+     });
+}
+
+int hiqsim_batch_build(hiqsim_batch_t* b)
+{
+     return guard([&] {
+          need(b, "batch");
+          b->b.build();
+     });
+}
+
+int hiqsim_batch_run(hiqsim_batch_t* b, double* device_ms)
+{
+     return guard([&] {
+          need(b, "batch");
+          const double ms = b->b.run();
+          if (device_ms) *device_ms = ms;
+     });
+}
+
+int hiqsim_batch_num_circuits(hiqsim_batch_t* b, int64_t* n)
+{
+     return guard([&] {
+          need(b, "batch");
+          need(n, "n");
+          *n = b->b.num_circuits();
+     });
+}
+
+int hiqsim_batch_num_qubits(hiqsim_batch_t* b, int64_t index, int* n_qubits)
+{
+     return guard([&] {
+          need(b, "batch");
+          need(n_qubits, "n_qubits");
+          *n_qubits = b->b.num_qubits(index);
+     });
+}
+
+int hiqsim_batch_get_state(hiqsim_batch_t* b, int64_t index, double* out)
+{
+     return guard([&] {
+          need(b, "batch");
+          need(out, "out");
+          b->b.get_state(index, reinterpret_cast<std::complex<double>*>(out));
+     });
+}
+
+int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len)
+{
+     return guard([&] {
+          need(b, "batch");
+          if (len) need(out, "out");
+          const hq::BatchStats& s = b->b.stats();
+          const uint64_t v[10] = {s.circuits,       s.passes,      s.wg_circuits,
+                                  s.lds_circuits,   s.layer_launches, s.arena_bytes,
+                                  s.matrix_bytes,   s.amp_passes,  static_cast<uint64_t>(s.wg_grid),
+                                  static_cast<uint64_t>(s.kmax)};
+          for (size_t i = 0; i < len && i < 10; ++i) out[i] = v[i];
+     });
+}
+
+int hiqsim_sample(hiqsim_t* h, uint32_t shots, uint64_t seed, uint64_t* out)
+{
+     return guard([&] {
+          need(h, "handle");
+          if (shots) need(out, "out");
+          h->eng.sample(shots, seed, out);
+     });
+}
+
+int hiqsim_server_create(int precision, int device, uint64_t state_bytes, uint64_t matrix_bytes,
+                         int passes_per_step, int max_active, hiqsim_server_t** out)
+{
+     return guard([&] {
+          need(out, "out");
+          if (precision != HIQSIM_C128 && precision != HIQSIM_C64)
+               throw std::runtime_error("precision must be HIQSIM_C128 or HIQSIM_C64");
+          *out = new hiqsim_server(static_cast<hq::Prec>(precision), device, state_bytes, matrix_bytes,
+                                   passes_per_step, max_active);
+     });
+}
+
+int hiqsim_server_destroy(hiqsim_server_t* s)
+{
+     return guard([&] { delete s; });
+}
+
+int hiqsim_server_set_widths(hiqsim_server_t* s, const int32_t* k_by_n, size_t len)
+{
+     return guard([&] {
+          need(s, "server");
+          if (len) need(k_by_n, "k_by_n");
+          s->s.set_widths(std::vector<int>(k_by_n, k_by_n + len));
+     });
+}
+
+int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
+                         const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, int kind, uint32_t shots, uint64_t seed,
+                         const uint64_t* xmask, const uint64_t* zmask, size_t nterms, int64_t* id)
+{
+     return guard([&] {
+          need(s, "server");
+          need(id, "id");
+          if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 (samples) or 1 (expectation)");
+          std::vector<uint64_t> X, Z;
+          if (kind == 1 && nterms) {
+               need(xmask, "xmask");
+               need(zmask, "zmask");
+               X.assign(xmask, xmask + nterms);
+               Z.assign(zmask, zmask + nterms);
+          }
+          *id = s->s.submit(n_qubits, batch_gates(ngates, mats, tgt_ids, tgt_off, ctrl_ids, ctrl_off),
+                            static_cast<hq::ServeKind>(kind), shots, seed, std::move(X), std::move(Z));
+     });
+}
+
+int hiqsim_server_step(hiqsim_server_t* s, int block, int64_t* completed, size_t cap,
+                       size_t* ncompleted)
+{
+     return guard([&] {
+          need(s, "server");
+          const auto done = s->s.step(block != 0);
+          if (cap) need(completed, "completed");
+          for (size_t i = 0; i < done.size() && i < cap; ++i) completed[i] = done[i];
+          if (ncompleted) *ncompleted = done.size();
+     });
+}
+
+int hiqsim_server_idle(hiqsim_server_t* s, int* idle)
+{
+     return guard([&] {
+          need(s, "server");
+          need(idle, "idle");
+          *idle = s->s.idle() ? 1 : 0;
+     });
+}
+
+int hiqsim_server_done(hiqsim_server_t* s, int64_t id, int* done)
+{
+     return guard([&] {
+          need(s, "server");
+          need(done, "done");
+          *done = s->s.done(id) ? 1 : 0;
+     });
+}
+
+int hiqsim_server_samples(hiqsim_server_t* s, int64_t id, uint64_t* out, size_t cap, size_t* n)
+{
+     return guard([&] {
+          need(s, "server");
+          const auto& v = s->s.samples(id);
+          if (cap) need(out, "out");
+          for (size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
+          if (n) *n = v.size();
+     });
+}
+
+int hiqsim_server_expectations(hiqsim_server_t* s, int64_t id, double* out, size_t cap_terms,
+                               size_t* n)
+{
+     return guard([&] {
+          need(s, "server");
+          const auto& v = s->s.expectations(id);
+          if (cap_terms) need(out, "out");
+          for (size_t i = 0; i < v.size() && i < cap_terms; ++i) {
+               out[2 * i] = v[i].real();
+               out[2 * i + 1] = v[i].imag();
+          }
+          if (n) *n = v.size();
+     });
+}
+
+int hiqsim_server_times(hiqsim_server_t* s, int64_t id, double* out)
+{
+     return guard([&] {
+          need(s, "server");
+          need(out, "out");
+          s->s.times(id, out, out + 1, out + 2);
+     });
+}
+
+int hiqsim_server_forget(hiqsim_server_t* s, int64_t id)
+{
+     return guard([&] {
+          need(s, "server");
+          s->s.forget(id);
+     });
+}
+
+int hiqsim_server_stats(hiqsim_server_t* s, uint64_t* out, size_t len)
+{
+     return guard([&] {
+          need(s, "server");
+          if (len) need(out, "out");
+          const hq::ServeStats& t = s->s.stats();
+          const uint64_t v[6] = {t.submitted, t.completed, t.steps, t.launches, t.peak_active,
+                                 t.peak_state_bytes};
+          for (size_t i = 0; i < len && i < 6; ++i) out[i] = v[i];
      });
 }
 

@@ -1,6 +1,8 @@
 // Host side of the single-GPU state-vector engine.  See engine.hpp.
 #include "engine.hpp"
 
+#include "batch.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -101,6 +103,7 @@ Engine::~Engine()
      if (d_blocks_) (void)hipFree(d_blocks_);
      if (h_blocks_) (void)hipHostFree(h_blocks_);
      if (h_stage_) (void)hipHostFree(h_stage_);
+     if (d_samples_) (void)hipFree(d_samples_);
      if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -725,7 +728,7 @@ void Engine::restore()
      locals_ = snap_locals_;
 }
 
-void Engine::compare_snapshot(double out[4])
+void Engine::compare_snapshot(double out[5])
 {
      DeviceScope ds(device_);
      if (!has_snap_ || n_snap_ != n_amps_)
@@ -735,8 +738,8 @@ void Engine::compare_snapshot(double out[4])
                           d_scratch_ + kReduceScratchDoubles, stream_),
            "compare");
      end_timed();
-     read_out(h_out_, 4);
-     for (int i = 0; i < 4; ++i) out[i] = h_out_[i];
+     read_out(h_out_, 5);
+     for (int i = 0; i < 5; ++i) out[i] = h_out_[i];
 }
 
 void Engine::init_random_state(uint64_t seed)
@@ -751,6 +754,50 @@ void Engine::init_random_state(uint64_t seed)
      begin_timed(kScale);
      check(launch_scale(d_state_, prec_, n_amps_, f, 0.0, stream_), "scale");
      end_timed();
+}
+
+void Engine::sample(uint32_t shots, uint64_t seed, uint64_t* out)
+{
+     DeviceScope ds(device_);
+     if (shots == 0) return;
+     if (shots > (1u << 20)) throw std::runtime_error("sample(): at most 2^20 shots");
+     if (!out) throw std::runtime_error("sample(): null output");
+     run();
+     int n = 0;
+     while ((uint64_t(1) << n) < n_amps_) ++n;
+     if (shots > samples_cap_) {
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          if (d_samples_) check(hipFree(d_samples_), "hipFree");
+          d_samples_ = nullptr;
+          check(hipMalloc(reinterpret_cast<void**>(&d_samples_), shots * sizeof(uint64_t)), "hipMalloc(samples)");
+          samples_cap_ = shots;
+     }
+     const uint32_t lb = sample_log_blocks(n);
+     BatchItem it{};
+     it.log_groups = static_cast<uint32_t>(n) - lb;
+     SampleJob job{};
+     job.seed = seed;
+     job.log_blocks = lb;
+     job.log_block_amps = static_cast<uint32_t>(n) - lb;
+     job.shots = shots;
+     // Descriptors go through the pinned output buffer into the output area
+     // of the scratch (both hold 2 * kMaxOuts doubles).
+     char* h = reinterpret_cast<char*>(h_out_);
+     std::memcpy(h, &it, sizeof(it));
+     std::memcpy(h + 256, &job, sizeof(job));
+     char* d = reinterpret_cast<char*>(d_scratch_ + kReduceScratchDoubles);
+     check(hipMemcpyAsync(d, h, 512, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(descriptors)");
+     begin_timed(kReduce);
+     check(launch_block_sums_multi(d_state_, prec_, reinterpret_cast<const BatchItem*>(d), 1,
+                                   uint64_t(1) << lb, d_blocks_, stream_),
+           "block_sums_multi");
+     check(launch_sample(d_state_, prec_, reinterpret_cast<const SampleJob*>(d + 256), 1, d_blocks_,
+                         d_samples_, stream_),
+           "sample");
+     end_timed();
+     check(hipMemcpyAsync(out, d_samples_, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_),
+           "hipMemcpyAsync(samples)");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 // ---------------------------------------------------------------------------

@@ -91,10 +91,14 @@ public:
      void restore();
      // Compares the state with the snapshot: {Re, Im of <snapshot|state>,
      // <snapshot|snapshot>, max |state - snapshot|}, accumulated in FP64.
-     void compare_snapshot(double out[4]);
+     void compare_snapshot(double out[5]);
      // Fills the state with a normalised complex Gaussian random vector
      // (Philox4x32-10 keyed by `seed`, one counter per amplitude).
      void init_random_state(uint64_t seed);
+     // `shots` computational-basis samples of the current state (index bit p
+     // = the qubit at position p), shot s drawn from Philox4x32-10 with
+     // counter s and key `seed`; the state is not changed.
+     void sample(uint32_t shots, uint64_t seed, uint64_t* out);
 
      // Pauli sums; mask bit i refers to ids[i].  Y is encoded as X and Z.
      cplx expectation(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
@@ -169,6 +173,8 @@ private:
      double* h_blocks_ = nullptr;   // pinned
      char* h_stage_ = nullptr;      // pinned staging for block scans
      size_t stage_bytes_ = 0;
+     uint64_t* d_samples_ = nullptr;  // sample outcomes
+     size_t samples_cap_ = 0;
 
      bool timing_ = false;
      std::vector<TimedLaunch> pending_;

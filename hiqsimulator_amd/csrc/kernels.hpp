@@ -74,7 +74,7 @@ hipError_t launch_permute(void* dst, const void* src, Prec prec, int n_qubits,
 // ---- reductions (FP64 accumulation; results land in device `out`) --------
 // Number of FP64 slots of scratch the reductions need.
 constexpr int kReduceBlocks = 1024;
-constexpr size_t kReduceScratchDoubles = 4 * kReduceBlocks + 16;
+constexpr size_t kReduceScratchDoubles = 5 * kReduceBlocks + 16;
 
 // out[0] = sum |psi[i]|^2 over i with (i & msk) == val.
 hipError_t launch_masked_norm(const void* psi, Prec prec, int n_qubits, uint64_t msk,
@@ -97,7 +97,7 @@ hipError_t launch_block_sums(const void* psi, Prec prec, uint64_t n, uint64_t nb
 // psi[i] = complex Gaussian N(0,1) + i N(0,1) from Philox4x32-10 keyed by
 // `seed` with counter i (not normalised).
 hipError_t launch_random_state(void* psi, Prec prec, uint64_t n, uint64_t seed, hipStream_t s);
-// out = {Re, Im of sum conj(ref) psi, sum |ref|^2, max |psi - ref|}.
+// out = {Re, Im of sum conj(ref) psi, sum |ref|^2, max |psi - ref|, sum |psi|^2}.
 hipError_t launch_compare(const void* psi, const void* ref, Prec prec, uint64_t n, double* scratch,
                           double* out, hipStream_t s);
 

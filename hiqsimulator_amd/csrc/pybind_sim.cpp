@@ -291,21 +291,34 @@ public:
      }
      py::dict compare_snapshot()
      {
-          double out[4];
+          double out[5];
           {
                py::gil_scoped_release r;
                ok(hiqsim_compare_snapshot(h_, out));
           }
           py::dict d;
           d["overlap"] = std::complex<double>(out[0], out[1]);
-          d["ref_norm"] = out[2];
+          d["ref_norm2"] = out[2];
           d["max_abs_diff"] = out[3];
+          d["state_norm2"] = out[4];
+          // |<ref|psi>|^2 / (<ref|ref> <psi|psi>)
+          d["fidelity"] = (out[0] * out[0] + out[1] * out[1]) / (out[2] * out[4]);
           return d;
      }
      void init_random_state(uint64_t seed)
      {
           py::gil_scoped_release r;
           ok(hiqsim_init_random_state(h_, seed));
+     }
+     py::array_t<uint64_t> sample(uint32_t shots, uint64_t seed)
+     {
+          py::array_t<uint64_t> a(static_cast<py::ssize_t>(shots));
+          uint64_t* d = a.mutable_data();
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_sample(h_, shots, seed, d));
+          }
+          return a;
      }
      void synchronize()
      {
@@ -409,6 +422,241 @@ private:
      bool fuse_;
 };
 
+using CArr = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>;
+
+// [(matrix, targets, controls), ...] in the CSR form of the batch entry points.
+struct GateCsr {
+     std::vector<double> mats;
+     std::vector<int64_t> tid, cid;
+     std::vector<size_t> toff{0}, coff{0};
+     size_t size() const { return toff.size() - 1; }
+};
+
+GateCsr gate_csr(const py::list& gates)
+{
+     GateCsr g;
+     for (auto item: gates) {
+          auto t = item.cast<py::sequence>();
+          if (py::len(t) != 3) throw std::runtime_error("gates must be (matrix, targets, controls)");
+          CArr m = t[0].cast<CArr>();
+          for (auto q: t[1].cast<py::sequence>()) g.tid.push_back(q.cast<int64_t>());
+          for (auto q: t[2].cast<py::sequence>()) g.cid.push_back(q.cast<int64_t>());
+          const size_t nt = g.tid.size() - g.toff.back();
+          if (nt == 0 || nt > 5) throw std::runtime_error("a gate needs 1..5 targets");
+          const size_t dd = (size_t(1) << nt) * (size_t(1) << nt);
+          if (static_cast<size_t>(m.size()) != dd)
+               throw std::runtime_error("gate matrix size does not match its targets");
+          const std::complex<double>* d = m.data();
+          for (size_t i = 0; i < dd; ++i) {
+               g.mats.push_back(d[i].real());
+               g.mats.push_back(d[i].imag());
+          }
+          g.toff.push_back(g.tid.size());
+          g.coff.push_back(g.cid.size());
+     }
+     return g;
+}
+
+// Many independent circuits run together (hiqsim_batch_*).
+class PyBatch {
+public:
+     PyBatch(const std::string& precision, int device)
+     {
+          int p;
+          if (precision == "c128" || precision == "complex128") p = HIQSIM_C128;
+          else if (precision == "c64" || precision == "complex64") p = HIQSIM_C64;
+          else throw std::runtime_error("precision must be 'c128' or 'c64'");
+          ok(hiqsim_batch_create(p, device, &h_));
+     }
+     ~PyBatch()
+     {
+          if (h_) hiqsim_batch_destroy(h_);
+     }
+     // gates: [(matrix, targets, controls), ...]
+     int64_t add_circuit(int n, const py::list& gates)
+     {
+          const GateCsr g = gate_csr(gates);
+          int64_t idx = -1;
+          ok(hiqsim_batch_add_circuit(h_, n, g.size(), g.mats.data(), g.tid.data(), g.toff.data(),
+                                      g.cid.data(), g.coff.data(), &idx));
+          return idx;
+     }
+     int64_t add_copies(int64_t index, int64_t count)
+     {
+          int64_t first = -1;
+          ok(hiqsim_batch_add_copies(h_, index, count, &first));
+          return first;
+     }
+     void set_plan(const std::map<int, int>& k_by_n, int wg_max_qubits)
+     {
+          std::vector<int32_t> v;
+          for (auto& kv: k_by_n) {
+               if (kv.first < 0 || kv.first > 64) throw std::runtime_error("qubit count out of range");
+               if (v.size() <= static_cast<size_t>(kv.first)) v.resize(kv.first + 1, 0);
+               v[kv.first] = kv.second;
+          }
+          ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits));
+     }
+     void build()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_batch_build(h_));
+     }
+     double run()
+     {
+          double ms = 0;
+          py::gil_scoped_release r;
+          ok(hiqsim_batch_run(h_, &ms));
+          return ms;
+     }
+     int64_t size() const
+     {
+          int64_t n = 0;
+          ok(hiqsim_batch_num_circuits(h_, &n));
+          return n;
+     }
+     int num_qubits(int64_t i) const
+     {
+          int n = 0;
+          ok(hiqsim_batch_num_qubits(h_, i, &n));
+          return n;
+     }
+     py::array_t<std::complex<double>> get_state(int64_t i) const
+     {
+          const int n = num_qubits(i);
+          py::array_t<std::complex<double>> a(static_cast<py::ssize_t>(1) << n);
+          double* d = reinterpret_cast<double*>(a.mutable_data());
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_batch_get_state(h_, i, d));
+          }
+          return a;
+     }
+     py::dict stats() const
+     {
+          uint64_t v[10] = {0};
+          ok(hiqsim_batch_stats(h_, v, 10));
+          static const char* names[10] = {"circuits",      "passes",        "wg_circuits",
+                                          "lds_circuits",  "layer_launches", "arena_bytes",
+                                          "matrix_bytes",  "amp_passes",    "wg_grid",
+                                          "kmax"};
+          py::dict d;
+          for (int i = 0; i < 10; ++i) d[names[i]] = v[i];
+          return d;
+     }
+
+private:
+     hiqsim_batch_t* h_ = nullptr;
+};
+
+// Request loop (hiqsim_server_*).
+class PyServer {
+public:
+     PyServer(const std::string& precision, int device, uint64_t state_bytes, uint64_t matrix_bytes,
+              int passes_per_step, int max_active)
+     {
+          int p;
+          if (precision == "c128" || precision == "complex128") p = HIQSIM_C128;
+          else if (precision == "c64" || precision == "complex64") p = HIQSIM_C64;
+          else throw std::runtime_error("precision must be 'c128' or 'c64'");
+          ok(hiqsim_server_create(p, device, state_bytes, matrix_bytes, passes_per_step, max_active, &h_));
+          max_active_ = static_cast<size_t>(max_active);
+     }
+     ~PyServer()
+     {
+          if (h_) hiqsim_server_destroy(h_);
+     }
+     void set_widths(const std::map<int, int>& k_by_n)
+     {
+          std::vector<int32_t> v;
+          for (auto& kv: k_by_n) {
+               if (kv.first < 0 || kv.first > 64) throw std::runtime_error("qubit count out of range");
+               if (v.size() <= static_cast<size_t>(kv.first)) v.resize(kv.first + 1, 0);
+               v[kv.first] = kv.second;
+          }
+          ok(hiqsim_server_set_widths(h_, v.data(), v.size()));
+     }
+     int64_t submit_samples(int n, const py::list& gates, uint32_t shots, uint64_t seed)
+     {
+          const GateCsr g = gate_csr(gates);
+          int64_t id = -1;
+          ok(hiqsim_server_submit(h_, n, g.size(), g.mats.data(), g.tid.data(), g.toff.data(),
+                                  g.cid.data(), g.coff.data(), 0, shots, seed, nullptr, nullptr, 0, &id));
+          return id;
+     }
+     int64_t submit_expectation(int n, const py::list& gates, const std::vector<uint64_t>& xmask,
+                                const std::vector<uint64_t>& zmask)
+     {
+          const GateCsr g = gate_csr(gates);
+          if (xmask.size() != zmask.size()) throw std::runtime_error("xmask and zmask differ in length");
+          int64_t id = -1;
+          ok(hiqsim_server_submit(h_, n, g.size(), g.mats.data(), g.tid.data(), g.toff.data(),
+                                  g.cid.data(), g.coff.data(), 1, 0, 0, xmask.data(), zmask.data(),
+                                  xmask.size(), &id));
+          return id;
+     }
+     std::vector<int64_t> step(bool block)
+     {
+          std::vector<int64_t> done(max_active_ + 16);
+          size_t n = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_server_step(h_, block ? 1 : 0, done.data(), done.size(), &n));
+          }
+          done.resize(std::min(n, done.size()));
+          return done;
+     }
+     bool idle() const
+     {
+          int v = 0;
+          ok(hiqsim_server_idle(h_, &v));
+          return v != 0;
+     }
+     bool done(int64_t id) const
+     {
+          int v = 0;
+          ok(hiqsim_server_done(h_, id, &v));
+          return v != 0;
+     }
+     py::array_t<uint64_t> samples(int64_t id) const
+     {
+          size_t n = 0;
+          ok(hiqsim_server_samples(h_, id, nullptr, 0, &n));
+          py::array_t<uint64_t> a(static_cast<py::ssize_t>(n));
+          ok(hiqsim_server_samples(h_, id, a.mutable_data(), n, &n));
+          return a;
+     }
+     py::array_t<std::complex<double>> expectations(int64_t id) const
+     {
+          size_t n = 0;
+          ok(hiqsim_server_expectations(h_, id, nullptr, 0, &n));
+          py::array_t<std::complex<double>> a(static_cast<py::ssize_t>(n));
+          ok(hiqsim_server_expectations(h_, id, reinterpret_cast<double*>(a.mutable_data()), n, &n));
+          return a;
+     }
+     py::tuple times(int64_t id) const
+     {
+          double t[3];
+          ok(hiqsim_server_times(h_, id, t));
+          return py::make_tuple(t[0], t[1], t[2]);
+     }
+     void forget(int64_t id) { ok(hiqsim_server_forget(h_, id)); }
+     py::dict stats() const
+     {
+          uint64_t v[6] = {0};
+          ok(hiqsim_server_stats(h_, v, 6));
+          static const char* names[6] = {"submitted", "completed", "steps", "launches", "peak_active",
+                                         "peak_state_bytes"};
+          py::dict d;
+          for (int i = 0; i < 6; ++i) d[names[i]] = v[i];
+          return d;
+     }
+
+private:
+     hiqsim_server_t* h_ = nullptr;
+     size_t max_active_ = 0;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_cppsim_amd, m)
@@ -452,11 +700,56 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("snapshot", &PySim::snapshot)
          .def("compare_snapshot", &PySim::compare_snapshot)
          .def("init_random_state", &PySim::init_random_state)
+         .def("sample", &PySim::sample, py::arg("shots"), py::arg("seed"))
          .def("restore", &PySim::restore)
          .def("synchronize", &PySim::synchronize)
          .def("set_timing", &PySim::set_timing)
          .def("kernel_time", &PySim::kernel_time)
          .def("state_size", &PySim::state_size);
+     m.def("plan_clusters",
+           [](const std::vector<std::vector<int64_t>>& gate,
+              const std::vector<std::vector<int64_t>>& ctrl, int cluster_size, int mode) {
+                CsrGates g(gate, ctrl, std::vector<bool>(gate.size(), false));
+                std::vector<int32_t> out(gate.size());
+                {
+                     py::gil_scoped_release r;
+                     ok(hiqsim_plan_clusters(g.gid.data(), g.goff.data(), g.cid.data(),
+                                             g.coff.data(), gate.size(), cluster_size, mode,
+                                             out.data()));
+                }
+                return out;
+           },
+           py::arg("gate"), py::arg("gate_ctrl"), py::arg("cluster_size"), py::arg("mode") = 0);
+     py::class_<PyServer>(m, "Server")
+         .def(py::init<const std::string&, int, uint64_t, uint64_t, int, int>(),
+              py::arg("precision") = "c64", py::arg("device") = 0,
+              py::arg("state_bytes") = uint64_t(32) << 30, py::arg("matrix_bytes") = uint64_t(2) << 30,
+              py::arg("passes_per_step") = 8, py::arg("max_active") = 1024)
+         .def("set_widths", &PyServer::set_widths)
+         .def("submit_samples", &PyServer::submit_samples, py::arg("n_qubits"), py::arg("gates"),
+              py::arg("shots") = 1024, py::arg("seed") = 0)
+         .def("submit_expectation", &PyServer::submit_expectation, py::arg("n_qubits"),
+              py::arg("gates"), py::arg("xmask"), py::arg("zmask"))
+         .def("step", &PyServer::step, py::arg("block") = false)
+         .def("idle", &PyServer::idle)
+         .def("done", &PyServer::done)
+         .def("samples", &PyServer::samples)
+         .def("expectations", &PyServer::expectations)
+         .def("times", &PyServer::times)
+         .def("forget", &PyServer::forget)
+         .def("stats", &PyServer::stats);
+     py::class_<PyBatch>(m, "Batch")
+         .def(py::init<const std::string&, int>(), py::arg("precision") = "c128",
+              py::arg("device") = 0)
+         .def("add_circuit", &PyBatch::add_circuit, py::arg("n_qubits"), py::arg("gates"))
+         .def("add_copies", &PyBatch::add_copies, py::arg("index"), py::arg("count"))
+         .def("set_plan", &PyBatch::set_plan, py::arg("k_by_n"), py::arg("wg_max_qubits") = 16)
+         .def("build", &PyBatch::build)
+         .def("run", &PyBatch::run)
+         .def("__len__", &PyBatch::size)
+         .def("num_qubits", &PyBatch::num_qubits)
+         .def("get_state", &PyBatch::get_state)
+         .def("stats", &PyBatch::stats);
      py::class_<PySwapScheduler>(m, "SwapScheduler")
          .def(py::init<std::vector<std::vector<int64_t>>, std::vector<std::vector<int64_t>>,
                        std::vector<bool>, int64_t, int, bool>())

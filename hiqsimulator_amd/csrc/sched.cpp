@@ -331,6 +331,184 @@ std::vector<int> schedule_cluster(const GateList& g, const std::vector<int64_t>&
      return c.run();
 }
 
+std::vector<int> plan_clusters(const GateList& g, int cluster_size, int mode)
+{
+     check_sizes(g);
+     if (mode != kPlanReference && mode != kPlanLookahead)
+          throw std::runtime_error("plan_clusters(): mode must be 0 (reference) or 1 (lookahead)");
+     if (cluster_size < 1) throw std::runtime_error("plan_clusters(): cluster_size must be >= 1");
+     const size_t ng = g.gate.size();
+     std::vector<int> cluster_of(ng, -1);
+     if (ng == 0) return cluster_of;
+     if (mode == kPlanReference) {
+          std::vector<int64_t> all;
+          for (size_t i = 0; i < ng; ++i) {
+               all.insert(all.end(), g.gate[i].begin(), g.gate[i].end());
+               all.insert(all.end(), g.ctrl[i].begin(), g.ctrl[i].end());
+          }
+          std::sort(all.begin(), all.end());
+          all.erase(std::unique(all.begin(), all.end()), all.end());
+          std::vector<size_t> remaining(ng);
+          for (size_t i = 0; i < ng; ++i) remaining[i] = i;
+          int cid = 0;
+          while (!remaining.empty()) {
+               GateList sub;
+               for (auto i: remaining) {
+                    sub.gate.push_back(g.gate[i]);
+                    sub.ctrl.push_back(g.ctrl[i]);
+                    sub.diag.push_back(false);
+               }
+               auto avail = schedule_cluster(sub, all, {}, cluster_size);
+               if (avail.empty()) throw std::runtime_error("plan_clusters(): no cluster can be formed");
+               std::vector<uint8_t> taken(remaining.size(), 0);
+               for (int a: avail) {
+                    taken[a] = 1;
+                    cluster_of[remaining[a]] = cid;
+               }
+               std::vector<size_t> next;
+               for (size_t j = 0; j < remaining.size(); ++j)
+                    if (!taken[j]) next.push_back(remaining[j]);
+               remaining.swap(next);
+               ++cid;
+          }
+          return cluster_of;
+     }
+     // Lookahead packing on dense bit masks of the qubit ids.
+     Positions pos(g, {}, {});
+     std::vector<msk_t> q(ng);
+     for (size_t i = 0; i < ng; ++i) q[i] = pos.mask(g.gate[i]) | pos.mask(g.ctrl[i]);
+     std::vector<size_t> remaining(ng);
+     for (size_t i = 0; i < ng; ++i) remaining[i] = i;
+     // Gates a fixed qubit set S can take from the front of `remaining`
+     // (a gate is takeable if it lies in S and touches no qubit of a gate
+     // skipped before it); the scan ends once every qubit of S is blocked.
+     auto score = [&](msk_t S) {
+          msk_t blocked = 0;
+          int n = 0;
+          for (size_t j = 0; j < remaining.size(); ++j) {
+               const msk_t m = q[remaining[j]];
+               if (!(m & blocked) && submask(m, S)) ++n;
+               else blocked |= m;
+               if (submask(S, blocked)) break;
+          }
+          return n;
+     };
+     auto better = [](int n, msk_t S, int best_n, msk_t best) {
+          return n > best_n || (n == best_n && popc(S) < popc(best));
+     };
+     const int kWindow = 8 * cluster_size + 16;  // leading gates whose qubits form the pool
+     const int kSeeds = 16;
+     auto pool_gates = [](size_t j, int window) { return j < static_cast<size_t>(window); };
+     int cid = 0;
+     while (!remaining.empty()) {
+          msk_t best = 0;
+          int best_n = -1;
+          msk_t pool = 0;
+          {
+               // Seeds: the first gates that are still takeable (no earlier
+               // gate shares a qubit).  Each seed grows one qubit at a time,
+               // adding the pool qubit that lets the most gates be taken.
+               msk_t blocked = 0;
+               std::vector<msk_t> seeds;
+               for (size_t j = 0; j < remaining.size() && pool_gates(j, kWindow); ++j) {
+                    const msk_t m0 = q[remaining[j]];
+                    pool |= m0;
+                    if (!(m0 & blocked) && static_cast<int>(seeds.size()) < kSeeds)
+                         seeds.push_back(m0);
+                    blocked |= m0;
+               }
+               auto offer = [&](msk_t S) {
+                    const int n = score(S);
+                    if (better(n, S, best_n, best)) {
+                         best_n = n;
+                         best = S;
+                    }
+               };
+               for (msk_t seed: seeds) {
+                    // (a) first-come: take gates in program order while they fit
+                    msk_t S = seed, bl = 0;
+                    for (size_t i = 0; i < remaining.size() && popc(S) < cluster_size; ++i) {
+                         const msk_t m = q[remaining[i]];
+                         if (!(m & bl) && popc(S | m) <= cluster_size) S |= m;
+                         else bl |= m;
+                         if (submask(S, bl)) break;
+                    }
+                    offer(S);
+                    // (b) best-add: grow by the pool qubit that takes most gates
+                    S = seed;
+                    while (popc(S) < cluster_size) {
+                         msk_t add = 0;
+                         int add_n = -1;
+                         for (msk_t in = pool & ~S; in; in &= in - 1) {
+                              const msk_t b = in & (~in + 1);
+                              const int nn = score(S | b);
+                              if (nn > add_n) {
+                                   add_n = nn;
+                                   add = b;
+                              }
+                         }
+                         if (!add) break;
+                         S |= add;
+                    }
+                    offer(S);
+               }
+          }
+          // Hill climb: replace one qubit of the set by one of the pool.
+          for (int round = 0; round < 8 && popc(best) <= cluster_size; ++round) {
+               msk_t cand_best = best;
+               int cand_n = best_n;
+               for (msk_t out = best; out; out &= out - 1) {
+                    const msk_t bo = out & (~out + 1);
+                    for (msk_t in = pool & ~best; in; in &= in - 1) {
+                         const msk_t S = (best & ~bo) | (in & (~in + 1));
+                         const int n = score(S);
+                         if (better(n, S, cand_n, cand_best)) {
+                              cand_n = n;
+                              cand_best = S;
+                         }
+                    }
+                    if (popc(best) > 1) {
+                         const msk_t S = best & ~bo;
+                         const int n = score(S);
+                         if (better(n, S, cand_n, cand_best)) {
+                              cand_n = n;
+                              cand_best = S;
+                         }
+                    }
+               }
+               if (popc(best) < cluster_size)
+                    for (msk_t in = pool & ~best; in; in &= in - 1) {
+                         const msk_t S = best | (in & (~in + 1));
+                         const int n = score(S);
+                         if (better(n, S, cand_n, cand_best)) {
+                              cand_n = n;
+                              cand_best = S;
+                         }
+                    }
+               if (cand_best == best) break;
+               best = cand_best;
+               best_n = cand_n;
+          }
+          // Take the gates of the chosen set.
+          msk_t blocked = 0;
+          std::vector<size_t> next;
+          next.reserve(remaining.size());
+          for (size_t j = 0; j < remaining.size(); ++j) {
+               const msk_t m = q[remaining[j]];
+               if (!(m & blocked) && submask(m, best)) cluster_of[remaining[j]] = cid;
+               else {
+                    blocked |= m;
+                    next.push_back(remaining[j]);
+               }
+          }
+          if (next.size() == remaining.size())
+               throw std::runtime_error("plan_clusters(): no gate could be scheduled");
+          remaining.swap(next);
+          ++cid;
+     }
+     return cluster_of;
+}
+
 std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals, bool fuse)
 {
      Swap s(g, num_splits, num_locals, fuse);

@@ -37,6 +37,18 @@ struct GateList {
 std::vector<int> schedule_cluster(const GateList& g, const std::vector<int64_t>& locals,
                                   const std::vector<int64_t>& globals, int cluster_size);
 
+// Whole-circuit cluster plans.  Returns, for every gate, the index of the
+// cluster it belongs to; clusters are numbered in execution order and a
+// cluster's gates run in program order.
+//  * kPlanReference: the GreedyScheduler loop (_greedyscheduler.py:119-137)
+//    calling schedule_cluster on the remaining gates until none are left;
+//  * kPlanLookahead: one scan per cluster: starting from the first remaining
+//    gate, take every later gate that fits in k qubits and touches no qubit
+//    of a gate skipped so far (the same commutation rule), stop when the
+//    cluster is full and blocked.  O(gates) per cluster.
+enum PlanMode : int { kPlanReference = 0, kPlanLookahead = 1 };
+std::vector<int> plan_clusters(const GateList& g, int cluster_size, int mode);
+
 // swap_scheduler.cpp:70-88.  Returns the qubit ids to be local.
 std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals,
                                    bool fuse);

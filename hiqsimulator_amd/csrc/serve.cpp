@@ -1,0 +1,566 @@
+// Host side of the request loop.  See serve.hpp.
+#include "serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+#include "fusion.hpp"
+#include "sched.hpp"
+
+namespace hq {
+namespace {
+
+constexpr int kDefaultWidth = 4;
+constexpr int kMaxWidth = 5;
+constexpr int kMaxServeQubits = 30;
+
+void check(hipError_t e, const char* what)
+{
+     if (e != hipSuccess) {
+          std::ostringstream os;
+          os << what << ": " << hipGetErrorString(e);
+          throw std::runtime_error(os.str());
+     }
+}
+
+struct DeviceScope {
+     explicit DeviceScope(int dev) { check(hipSetDevice(dev), "hipSetDevice"); }
+};
+
+double now_s()
+{
+     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Blocks of one item array laid out back to back (block_start ascending).
+struct ItemList {
+     std::vector<BatchItem> items;
+     uint64_t blocks = 0;
+     void add(BatchItem it, uint64_t nblocks)
+     {
+          it.block_start = blocks;
+          blocks += nblocks;
+          items.push_back(it);
+     }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+RangeAlloc::RangeAlloc(uint64_t capacity)
+{
+     if (capacity) free_[0] = capacity;
+     free_total_ = capacity;
+}
+
+bool RangeAlloc::alloc(uint64_t size, uint64_t align, uint64_t* off)
+{
+     if (size == 0) size = 1;
+     for (auto it = free_.begin(); it != free_.end(); ++it) {
+          const uint64_t start = (it->first + align - 1) / align * align;
+          const uint64_t end = it->first + it->second;
+          if (start + size > end) continue;
+          const uint64_t f_off = it->first, f_end = end;
+          free_.erase(it);
+          if (start > f_off) free_[f_off] = start - f_off;
+          if (start + size < f_end) free_[start + size] = f_end - (start + size);
+          free_total_ -= size;
+          *off = start;
+          return true;
+     }
+     return false;
+}
+
+void RangeAlloc::release(uint64_t off, uint64_t size)
+{
+     if (size == 0) size = 1;
+     auto next = free_.lower_bound(off);
+     uint64_t start = off, end = off + size;
+     if (next != free_.begin()) {
+          auto prev = std::prev(next);
+          if (prev->first + prev->second == start) {
+               start = prev->first;
+               free_.erase(prev);
+          }
+     }
+     if (next != free_.end() && next->first == end) {
+          end = next->first + next->second;
+          free_.erase(next);
+     }
+     free_[start] = end - start;
+     free_total_ += size;
+}
+
+// ---------------------------------------------------------------------------
+Server::Server(Prec prec, int device, uint64_t state_bytes, uint64_t matrix_bytes,
+               int passes_per_step, int max_active)
+    : prec_(prec), device_(device), pps_(passes_per_step), max_active_(static_cast<size_t>(max_active))
+{
+     if (passes_per_step < 1 || passes_per_step > 64)
+          throw std::runtime_error("Server(): passes_per_step must be 1..64");
+     if (max_active < 1 || max_active > 4096) throw std::runtime_error("Server(): max_active must be 1..4096");
+     int ndev = 0;
+     check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+     if (device < 0 || device >= ndev)
+          throw std::runtime_error("Server(): no HIP device " + std::to_string(device));
+     DeviceScope ds(device_);
+     const size_t ab = amp_bytes(prec_);
+     state_cap_amps_ = state_bytes / ab;
+     const uint64_t mat_entries = matrix_bytes / ab;
+     if (state_cap_amps_ < 2 || mat_entries < 4) throw std::runtime_error("Server(): pools too small");
+     size_t free_b = 0, total_b = 0;
+     check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+     if (state_bytes + matrix_bytes + (size_t(2) << 30) > free_b)
+          throw std::runtime_error("Server(): pools exceed free device memory");
+     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+     check(hipMalloc(&d_states_, state_cap_amps_ * ab), "hipMalloc(states)");
+     check(hipMalloc(&d_mats_, mat_entries * ab), "hipMalloc(matrices)");
+     check(hipMalloc(reinterpret_cast<void**>(&d_scratch_), kReduceScratchDoubles * sizeof(double)),
+           "hipMalloc(scratch)");
+     state_pool_ = RangeAlloc(state_cap_amps_);
+     mat_pool_ = RangeAlloc(mat_entries);
+     for (auto& s: slots_) check(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
+}
+
+Server::~Server()
+{
+     (void)hipSetDevice(device_);
+     (void)hipStreamSynchronize(stream_);
+     for (auto& s: slots_) {
+          if (s.h_desc) (void)hipHostFree(s.h_desc);
+          if (s.h_out) (void)hipHostFree(s.h_out);
+          if (s.d_desc) (void)hipFree(s.d_desc);
+          if (s.d_out) (void)hipFree(s.d_out);
+          (void)hipEventDestroy(s.ev);
+     }
+     for (void* p: {d_states_, d_mats_, static_cast<void*>(d_sums_), static_cast<void*>(d_scratch_)})
+          if (p) (void)hipFree(p);
+     (void)hipStreamDestroy(stream_);
+}
+
+void Server::set_widths(const std::vector<int>& k_by_n)
+{
+     for (int k: k_by_n)
+          if (k < 0 || k > kMaxWidth) throw std::runtime_error("Server: fusion width must be 0..5");
+     k_by_n_ = k_by_n;
+}
+
+int Server::fusion_width(int n) const
+{
+     const int k = (n >= 0 && static_cast<size_t>(n) < k_by_n_.size()) ? k_by_n_[n] : 0;
+     return k > 0 ? k : kDefaultWidth;
+}
+
+int64_t Server::submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint32_t shots,
+                       uint64_t seed, std::vector<uint64_t> xmask, std::vector<uint64_t> zmask)
+{
+     if (n < 1 || n > kMaxServeQubits)
+          throw std::runtime_error("Server: requests must have 1.." + std::to_string(kMaxServeQubits) + " qubits");
+     if ((uint64_t(1) << n) > state_cap_amps_)
+          throw std::runtime_error("Server: the request's state does not fit the state pool");
+     if (kind == ServeKind::Samples && (shots == 0 || shots > (1u << 20)))
+          throw std::runtime_error("Server: shots must be 1..2^20");
+     if (kind == ServeKind::Expectation) {
+          if (xmask.size() != zmask.size() || xmask.empty())
+               throw std::runtime_error("Server: expectation needs one x and one z mask per term");
+          const uint64_t lim = n >= 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
+          for (size_t t = 0; t < xmask.size(); ++t)
+               if ((xmask[t] | zmask[t]) & ~lim) throw std::runtime_error("Server: Pauli term acts beyond the register");
+     }
+     // Validation and planning as for a batch circuit.
+     for (size_t gi = 0; gi < gates.size(); ++gi) {
+          const auto& g = gates[gi];
+          const std::string where = "Server: gate " + std::to_string(gi) + ": ";
+          if (g.targets.empty() || g.targets.size() + g.ctrls.size() > static_cast<size_t>(kMaxWidth))
+               throw std::runtime_error(where + "needs 1..5 qubits with its controls");
+          const size_t dim = size_t(1) << g.targets.size();
+          if (g.m.size() != dim * dim) throw std::runtime_error(where + "matrix size does not match targets");
+          std::vector<int64_t> all = g.targets;
+          all.insert(all.end(), g.ctrls.begin(), g.ctrls.end());
+          for (auto q: all)
+               if (q < 0 || q >= n) throw std::runtime_error(where + "qubit id out of range");
+          std::sort(all.begin(), all.end());
+          if (std::adjacent_find(all.begin(), all.end()) != all.end())
+               throw std::runtime_error(where + "repeated qubit");
+     }
+     Req r;
+     r.n = n;
+     r.kind = kind;
+     r.shots = shots;
+     r.seed = seed;
+     r.xmask = std::move(xmask);
+     r.zmask = std::move(zmask);
+     r.t_submit = now_s();
+     const size_t ab = amp_bytes(prec_);
+     if (!gates.empty()) {
+          GateList gl;
+          for (const auto& g: gates) {
+               gl.gate.push_back(g.targets);
+               gl.ctrl.push_back(g.ctrls);
+               gl.diag.push_back(false);
+          }
+          const std::vector<int> cluster_of = plan_clusters(gl, fusion_width(n), kPlanLookahead);
+          const int nc = *std::max_element(cluster_of.begin(), cluster_of.end()) + 1;
+          std::vector<std::vector<int>> members(nc);
+          for (size_t i = 0; i < gates.size(); ++i) members[cluster_of[i]].push_back(static_cast<int>(i));
+          std::vector<std::vector<cplx>> mats;
+          for (const auto& mem: members) {
+               Fusion f;
+               for (int gi: mem) {
+                    const BatchGate& g = gates[gi];
+                    Mat m(size_t(1) << g.targets.size());
+                    m.a = g.m;
+                    std::vector<int64_t> ids = g.targets;
+                    add_controls(m, ids, g.ctrls);
+                    f.insert(std::move(m), false, std::move(ids), {});
+               }
+               Mat M;
+               std::vector<int64_t> ids, ctrls;
+               bool diag = false;
+               f.perform(M, ids, ctrls, diag);
+               if (ids.size() > static_cast<size_t>(kMaxWidth) || !ctrls.empty())
+                    throw std::runtime_error("Server: a fused cluster spans more than 5 qubits");
+               Pass p{static_cast<int>(ids.size()), {0, 0, 0, 0, 0}};
+               for (size_t l = 0; l < ids.size(); ++l) p.pos[l] = static_cast<uint8_t>(ids[l]);
+               r.passes.push_back(p);
+               r.pass_mat.push_back(r.mat_entries);
+               r.mat_entries += M.a.size();
+               mats.push_back(std::move(M.a));
+          }
+          r.mats.resize(r.mat_entries * ab);
+          uint64_t o = 0;
+          for (const auto& m: mats) {
+               for (size_t i = 0; i < m.size(); ++i, ++o) {
+                    if (prec_ == Prec::C128) {
+                         double* d = reinterpret_cast<double*>(r.mats.data()) + 2 * o;
+                         d[0] = m[i].real();
+                         d[1] = m[i].imag();
+                    }
+                    else {
+                         float* d = reinterpret_cast<float*>(r.mats.data()) + 2 * o;
+                         d[0] = static_cast<float>(m[i].real());
+                         d[1] = static_cast<float>(m[i].imag());
+                    }
+               }
+          }
+     }
+     const int64_t id = next_id_++;
+     reqs_.emplace(id, std::move(r));
+     waiting_.push_back(id);
+     ++stats_.submitted;
+     return id;
+}
+
+void Server::ensure_slot(Slot& s, size_t desc_bytes, size_t out_bytes)
+{
+     if (desc_bytes > s.desc_cap) {
+          const size_t cap = std::max(desc_bytes, s.desc_cap * 2);
+          if (s.h_desc) check(hipHostFree(s.h_desc), "hipHostFree");
+          if (s.d_desc) check(hipFree(s.d_desc), "hipFree");
+          check(hipHostMalloc(reinterpret_cast<void**>(&s.h_desc), cap, hipHostMallocDefault), "hipHostMalloc");
+          check(hipMalloc(reinterpret_cast<void**>(&s.d_desc), cap), "hipMalloc");
+          s.desc_cap = cap;
+     }
+     if (out_bytes > s.out_cap) {
+          const size_t cap = std::max(out_bytes, s.out_cap * 2);
+          if (s.h_out) check(hipHostFree(s.h_out), "hipHostFree");
+          if (s.d_out) check(hipFree(s.d_out), "hipFree");
+          check(hipHostMalloc(reinterpret_cast<void**>(&s.h_out), cap, hipHostMallocDefault), "hipHostMalloc");
+          check(hipMalloc(reinterpret_cast<void**>(&s.d_out), cap), "hipMalloc");
+          s.out_cap = cap;
+     }
+}
+
+void Server::harvest(bool wait_oldest, std::vector<int64_t>& completed)
+{
+     while (!inflight_.empty()) {
+          Slot& s = slots_[inflight_.front()];
+          if (wait_oldest) {
+               check(hipEventSynchronize(s.ev), "hipEventSynchronize");
+               wait_oldest = false;
+          }
+          else {
+               const hipError_t q = hipEventQuery(s.ev);
+               if (q == hipErrorNotReady) return;
+               check(q, "hipEventQuery");
+          }
+          const double t = now_s();
+          for (const auto& fo: s.finishing) {
+               Req& r = reqs_.at(fo.first);
+               const char* src = s.h_out + fo.second;
+               if (r.kind == ServeKind::Samples) {
+                    r.res_samples.resize(r.shots);
+                    std::memcpy(r.res_samples.data(), src, r.shots * sizeof(uint64_t));
+               }
+               else {
+                    // <X^x Z^z> from the kernel times i^{|x & z|} (Y = i X Z)
+                    static const std::complex<double> ip[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+                    r.res_exp.resize(r.xmask.size());
+                    const double* d = reinterpret_cast<const double*>(src);
+                    for (size_t i = 0; i < r.res_exp.size(); ++i)
+                         r.res_exp[i] = ip[__builtin_popcountll(r.xmask[i] & r.zmask[i]) & 3] *
+                                        std::complex<double>(d[2 * i], d[2 * i + 1]);
+               }
+               r.state = 3;
+               r.t_done = t;
+               completed.push_back(fo.first);
+               ++stats_.completed;
+          }
+          s.finishing.clear();
+          s.busy = false;
+          inflight_.pop_front();
+     }
+}
+
+std::vector<int64_t> Server::step(bool block)
+{
+     DeviceScope ds(device_);
+     std::vector<int64_t> completed;
+     harvest(false, completed);
+     const size_t ab = amp_bytes(prec_);
+
+     // Admission in arrival order; a request that does not fit lets later
+     // ones try.
+     std::vector<int64_t> admitted;
+     for (auto it = waiting_.begin(); it != waiting_.end() && active_.size() < max_active_;) {
+          Req& r = reqs_.at(*it);
+          uint64_t so = 0, mo = 0;
+          if (!state_pool_.alloc(uint64_t(1) << r.n, 256 / ab, &so)) {
+               ++it;
+               continue;
+          }
+          if (r.mat_entries && !mat_pool_.alloc(r.mat_entries, 64, &mo)) {
+               state_pool_.release(so, uint64_t(1) << r.n);
+               ++it;
+               continue;
+          }
+          r.state_off = so;
+          r.mat_off = mo;
+          r.state = 1;
+          r.t_admit = now_s();
+          admitted.push_back(*it);
+          active_.push_back(*it);
+          it = waiting_.erase(it);
+     }
+     stats_.peak_active = std::max<uint64_t>(stats_.peak_active, active_.size());
+     stats_.peak_state_bytes = std::max<uint64_t>(stats_.peak_state_bytes,
+                                                  (state_cap_amps_ - state_pool_.free_total()) * ab);
+     if (active_.empty()) {
+          if (block && !inflight_.empty()) harvest(true, completed);
+          return completed;
+     }
+     // A free slot: wait for the oldest step when both are in flight.
+     if (inflight_.size() >= static_cast<size_t>(kServeSlots)) harvest(true, completed);
+     const int si = next_slot_;
+     next_slot_ = (next_slot_ + 1) % kServeSlots;
+     Slot& slot = slots_[si];
+
+     // Descriptors of this step.
+     ItemList init;
+     for (int64_t id: admitted) {
+          const Req& r = reqs_.at(id);
+          BatchItem it{};
+          it.state_off = r.state_off;
+          it.log_groups = static_cast<uint32_t>(r.n);
+          init.add(it, ((uint64_t(1) << r.n) + kInitAmpsPerBlock - 1) / kInitAmpsPerBlock);
+     }
+     std::vector<ItemList> layers(pps_);
+     int kmax = 1;
+     std::vector<int64_t> finishing;
+     for (int64_t id: active_) {
+          Req& r = reqs_.at(id);
+          for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
+               const Pass& p = r.passes[r.next + l];
+               BatchItem it{};
+               it.state_off = r.state_off;
+               it.mat_off = r.mat_off + r.pass_mat[r.next + l];
+               it.log_groups = static_cast<uint32_t>(r.n - p.k);
+               std::memcpy(it.pos, p.pos, 5);
+               it.k = static_cast<uint8_t>(p.k);
+               kmax = std::max(kmax, p.k);
+               layers[l].add(it, ((uint64_t(1) << it.log_groups) + kBatchLayerBlock - 1) / kBatchLayerBlock);
+          }
+          if (r.next + pps_ >= r.passes.size()) finishing.push_back(id);
+     }
+     // Outputs of the requests that finish in this step.
+     ItemList sums;
+     std::vector<SampleJob> jobs;
+     std::vector<std::pair<int64_t, size_t>> outs;
+     size_t out_bytes = 0;
+     uint64_t sums_needed = 0;
+     for (int64_t id: finishing) {
+          Req& r = reqs_.at(id);
+          outs.push_back({id, out_bytes});
+          if (r.kind == ServeKind::Samples) {
+               const uint32_t lb = sample_log_blocks(r.n);
+               BatchItem it{};
+               it.state_off = r.state_off;
+               it.mat_off = sums_needed;
+               it.log_groups = static_cast<uint32_t>(r.n) - lb;
+               sums.add(it, uint64_t(1) << lb);
+               SampleJob j{};
+               j.state_off = r.state_off;
+               j.sums_off = sums_needed;
+               j.out_off = out_bytes / sizeof(uint64_t);
+               j.seed = r.seed;
+               j.log_blocks = lb;
+               j.log_block_amps = static_cast<uint32_t>(r.n) - lb;
+               j.shots = r.shots;
+               jobs.push_back(j);
+               sums_needed += uint64_t(1) << lb;
+               out_bytes += align_up(r.shots * sizeof(uint64_t), 256);
+          }
+          else {
+               out_bytes += align_up(r.xmask.size() * 2 * sizeof(double), 256);
+          }
+     }
+     if (sums_needed > sums_cap_) {
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          if (d_sums_) check(hipFree(d_sums_), "hipFree");
+          sums_cap_ = std::max<uint64_t>(sums_needed, 2 * sums_cap_);
+          check(hipMalloc(reinterpret_cast<void**>(&d_sums_), sums_cap_ * sizeof(double)), "hipMalloc(sums)");
+     }
+     // Layout in the slot's staging buffer.
+     const size_t o_init = 0;
+     size_t o = align_up(init.items.size() * sizeof(BatchItem), 256);
+     std::vector<size_t> o_layer(pps_);
+     for (int l = 0; l < pps_; ++l) {
+          o_layer[l] = o;
+          o += align_up(layers[l].items.size() * sizeof(BatchItem), 256);
+     }
+     const size_t o_sums = o;
+     o += align_up(sums.items.size() * sizeof(BatchItem), 256);
+     const size_t o_jobs = o;
+     o += align_up(jobs.size() * sizeof(SampleJob), 256);
+     const size_t desc_bytes = std::max<size_t>(o, 256);
+     ensure_slot(slot, desc_bytes, std::max<size_t>(out_bytes, 256));
+     std::memcpy(slot.h_desc + o_init, init.items.data(), init.items.size() * sizeof(BatchItem));
+     for (int l = 0; l < pps_; ++l)
+          std::memcpy(slot.h_desc + o_layer[l], layers[l].items.data(),
+                      layers[l].items.size() * sizeof(BatchItem));
+     std::memcpy(slot.h_desc + o_sums, sums.items.data(), sums.items.size() * sizeof(BatchItem));
+     std::memcpy(slot.h_desc + o_jobs, jobs.data(), jobs.size() * sizeof(SampleJob));
+
+     // Matrices of the newly admitted requests, then the step's launches.
+     for (int64_t id: admitted) {
+          const Req& r = reqs_.at(id);
+          if (r.mat_entries)
+               check(hipMemcpyAsync(static_cast<char*>(d_mats_) + r.mat_off * ab, r.mats.data(),
+                                    r.mat_entries * ab, hipMemcpyHostToDevice, stream_),
+                     "hipMemcpyAsync(matrices)");
+     }
+     check(hipMemcpyAsync(slot.d_desc, slot.h_desc, desc_bytes, hipMemcpyHostToDevice, stream_),
+           "hipMemcpyAsync(descriptors)");
+     auto d_items = [&](size_t off) { return reinterpret_cast<const BatchItem*>(slot.d_desc + off); };
+     if (!init.items.empty()) {
+          check(launch_init_states(d_states_, prec_, d_items(o_init), static_cast<int>(init.items.size()),
+                                   init.blocks, stream_),
+                "launch_init_states");
+          ++stats_.launches;
+     }
+     for (int l = 0; l < pps_; ++l) {
+          if (layers[l].items.empty()) continue;
+          check(launch_serve_layer(d_states_, prec_, d_mats_, d_items(o_layer[l]),
+                                   static_cast<int>(layers[l].items.size()), layers[l].blocks, kmax,
+                                   stream_),
+                "launch_serve_layer");
+          ++stats_.launches;
+     }
+     if (!jobs.empty()) {
+          check(launch_block_sums_multi(d_states_, prec_, d_items(o_sums), static_cast<int>(sums.items.size()),
+                                        sums.blocks, d_sums_, stream_),
+                "launch_block_sums_multi");
+          check(launch_sample(d_states_, prec_, reinterpret_cast<const SampleJob*>(slot.d_desc + o_jobs),
+                              static_cast<int>(jobs.size()), d_sums_,
+                              reinterpret_cast<uint64_t*>(slot.d_out), stream_),
+                "launch_sample");
+          stats_.launches += 2;
+     }
+     for (const auto& fo: outs) {
+          const Req& r = reqs_.at(fo.first);
+          if (r.kind != ServeKind::Expectation) continue;
+          const void* psi = static_cast<const char*>(d_states_) + r.state_off * ab;
+          double* out = reinterpret_cast<double*>(slot.d_out + fo.second);
+          for (size_t t = 0; t < r.xmask.size(); ++t) {
+               check(launch_pauli_expectation(psi, prec_, uint64_t(1) << r.n, r.xmask[t], r.zmask[t],
+                                              d_scratch_, out + 2 * t, stream_),
+                     "launch_pauli_expectation");
+               ++stats_.launches;
+          }
+     }
+     if (out_bytes)
+          check(hipMemcpyAsync(slot.h_out, slot.d_out, out_bytes, hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync(outputs)");
+     check(hipEventRecord(slot.ev, stream_), "hipEventRecord");
+     slot.busy = true;
+     slot.finishing = outs;
+     inflight_.push_back(si);
+     ++stats_.steps;
+
+     // Progress; finished requests give their memory back (later launches on
+     // the stream run after this step's outputs).
+     std::vector<int64_t> still;
+     for (int64_t id: active_) {
+          Req& r = reqs_.at(id);
+          r.next = std::min(r.passes.size(), r.next + static_cast<size_t>(pps_));
+          if (std::find(finishing.begin(), finishing.end(), id) != finishing.end()) {
+               r.state = 2;
+               state_pool_.release(r.state_off, uint64_t(1) << r.n);
+               if (r.mat_entries) mat_pool_.release(r.mat_off, r.mat_entries);
+               std::vector<char>().swap(r.mats);
+          }
+          else {
+               still.push_back(id);
+          }
+     }
+     active_.swap(still);
+     if (block && active_.empty() && waiting_.empty()) harvest(true, completed);
+     return completed;
+}
+
+bool Server::done(int64_t id) const
+{
+     auto it = reqs_.find(id);
+     if (it == reqs_.end()) throw std::runtime_error("Server: unknown request " + std::to_string(id));
+     return it->second.state == 3;
+}
+
+const std::vector<uint64_t>& Server::samples(int64_t id) const
+{
+     if (!done(id)) throw std::runtime_error("Server: request " + std::to_string(id) + " is not done");
+     const Req& r = reqs_.at(id);
+     if (r.kind != ServeKind::Samples) throw std::runtime_error("Server: request has no samples");
+     return r.res_samples;
+}
+
+const std::vector<std::complex<double>>& Server::expectations(int64_t id) const
+{
+     if (!done(id)) throw std::runtime_error("Server: request " + std::to_string(id) + " is not done");
+     const Req& r = reqs_.at(id);
+     if (r.kind != ServeKind::Expectation) throw std::runtime_error("Server: request has no expectations");
+     return r.res_exp;
+}
+
+void Server::times(int64_t id, double* t_submit, double* t_admit, double* t_done) const
+{
+     auto it = reqs_.find(id);
+     if (it == reqs_.end()) throw std::runtime_error("Server: unknown request " + std::to_string(id));
+     *t_submit = it->second.t_submit;
+     *t_admit = it->second.t_admit;
+     *t_done = it->second.t_done;
+}
+
+void Server::forget(int64_t id)
+{
+     auto it = reqs_.find(id);
+     if (it == reqs_.end()) return;
+     if (it->second.state != 3) throw std::runtime_error("Server: request is still running");
+     reqs_.erase(it);
+}
+
+}  // namespace hq

@@ -14,15 +14,29 @@ This module runs the same loop on plain gate lists ``(matrix, targets,
 controls)`` so that planning can happen ahead of time (outside a timed
 region) and independently of ProjectQ.
 """
-from . import ClusterScheduler
+from . import ClusterScheduler, _cppsim_amd
+
+PLAN_MODES = {"reference": 0, "lookahead": 1}
 
 
-def plan_clusters(gates, local_ids, cluster_size):
+def plan_clusters(gates, local_ids, cluster_size, mode="reference"):
     """Return the fused clusters as lists of gate indices, in execution order.
 
     Within a cluster the indices are ascending (program order); gates are
-    only moved past gates they share no qubit with.
+    only moved past gates they share no qubit with.  ``mode="reference"``
+    runs the GreedyScheduler loop below; ``mode="lookahead"`` the native
+    one-scan packer (same commutation rule, a few percent more clusters,
+    orders of magnitude faster on long circuits; every qubit local).
     """
+    if mode == "lookahead":
+        cluster_of = _cppsim_amd.plan_clusters([list(g[1]) for g in gates],
+                                               [list(g[2]) for g in gates], int(cluster_size), 1)
+        clusters = [[] for _ in range(max(cluster_of) + 1 if cluster_of else 0)]
+        for i, c in enumerate(cluster_of):
+            clusters[c].append(i)
+        return clusters
+    if mode != "reference":
+        raise ValueError("mode must be 'reference' or 'lookahead'")
     remaining = list(range(len(gates)))
     clusters = []
     local_ids = list(local_ids)
@@ -40,11 +54,11 @@ def plan_clusters(gates, local_ids, cluster_size):
     return clusters
 
 
-def apply_scheduled(sim, gates, cluster_size, local_ids=None):
+def apply_scheduled(sim, gates, cluster_size, local_ids=None, mode="reference"):
     """Apply a gate list cluster by cluster (one run() per cluster)."""
     if local_ids is None:
         local_ids = sim.get_qubits_ids()
-    for cluster in plan_clusters(gates, local_ids, cluster_size):
+    for cluster in plan_clusters(gates, local_ids, cluster_size, mode):
         for i in cluster:
             m, t, c = gates[i]
             sim.apply_controlled_gate(m, t, c)

@@ -111,11 +111,19 @@ int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps);
 int hiqsim_snapshot_state(hiqsim_t* h);
 int hiqsim_restore_state(hiqsim_t* h);
 /* Precision report against the snapshot (no reference counterpart):
- * out = {Re, Im of <snapshot|state>, <snapshot|snapshot>, max |state - snapshot|}. */
+ * out = {Re, Im of <snapshot|state>, <snapshot|snapshot>, max |state - snapshot|,
+ *        <state|state>} (out holds 5 doubles, accumulated in FP64). */
 int hiqsim_compare_snapshot(hiqsim_t* h, double* out);
 /* Normalised complex Gaussian random state (Philox4x32-10 keyed by seed, one
  * counter per amplitude); no reference counterpart. */
 int hiqsim_init_random_state(hiqsim_t* h, uint64_t seed);
+/* `shots` computational-basis samples of the state without collapsing it
+ * (pending gates are run first): out[s] = sampled index, bit p = the qubit
+ * hiqsim_get_qubits_ids() lists at p.  Shot s draws u from Philox4x32-10
+ * (counter s, key seed) and takes the first index whose running
+ * probability exceeds u * norm.  No reference counterpart: the reference
+ * samples by repeated MeasureQubits + collapse (SimulatorMPI.cpp:897-1008). */
+int hiqsim_sample(hiqsim_t* h, uint32_t shots, uint64_t seed, uint64_t* out);
 
 /* get_expectation_value (_simulator_mpi.py:148-183; unbound in the
  * reference): sum_t coeff[t] <psi|P_t|psi>.  Bit i of xmask[t]/zmask[t] refers
@@ -166,6 +174,96 @@ int hiqsim_schedule_swap(const int64_t* gate_ids, const size_t* gate_off, const 
                          const size_t* ctrl_off, const uint8_t* diag, size_t ngates,
                          int64_t num_splits, int num_locals, int fuse, int64_t* out,
                          size_t* nout);
+
+/* Whole-circuit cluster plan: cluster_of[g] = index of gate g's cluster
+ * (execution order).  mode 0 runs the reference GreedyScheduler loop over
+ * ClusterScheduler (_greedyscheduler.py:119-137); mode 1 the one-scan
+ * lookahead packer (no reference counterpart; same commutation rule). */
+int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, size_t ngates, int cluster_size, int mode,
+                         int32_t* cluster_of);
+
+/* ---- batches of independent circuits ------------------------------------
+ * No counterpart in the reference, which simulates one circuit per
+ * SimulatorMPI object.  Each circuit of a batch starts from |0...0> on its
+ * own qubits 0..n-1 and its final state equals what a SimulatorMPI (one
+ * rank) holds after apply_controlled_gate for every gate and run()
+ * (SimulatorMPI.cpp:407-436): the gates are cut into fused clusters of at
+ * most k qubits each (k per qubit count, hiqsim_batch_set_plan) and applied
+ * as dense passes.  All circuits of a batch run in one graph launch. */
+typedef struct hiqsim_batch hiqsim_batch_t;
+
+int hiqsim_batch_create(int precision, int device, hiqsim_batch_t** out);
+int hiqsim_batch_destroy(hiqsim_batch_t* b);
+/* Adds a circuit of n_qubits (1..30) qubits and ngates gates; returns its
+ * index in *index.  Gate g acts on targets tgt_ids[tgt_off[g]..tgt_off[g+1])
+ * with controls ctrl_ids[ctrl_off[g]..ctrl_off[g+1]) (all must be 1); its
+ * 2^t x 2^t matrix follows the previous gate's in mats (interleaved
+ * complex128, row-major).  A gate with its controls spans at most 5 qubits. */
+int hiqsim_batch_add_circuit(hiqsim_batch_t* b, int n_qubits, size_t ngates, const double* mats,
+                             const int64_t* tgt_ids, const size_t* tgt_off,
+                             const int64_t* ctrl_ids, const size_t* ctrl_off, int64_t* index);
+/* Appends `count` copies of circuit `index`; *first = the first copy's index. */
+int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int64_t* first);
+/* k_by_n[n] = fusion width (1..5, 0 = default 4) for circuits of n qubits;
+ * circuits of more than wg_max_qubits (0..24) qubits are advanced by
+ * layered many-workgroup launches, the others each by one workgroup of a
+ * persistent launch. */
+int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits);
+/* Plans, fuses, uploads and captures the launch graph. */
+int hiqsim_batch_build(hiqsim_batch_t* b);
+/* Runs every circuit from |0...0> (graph replay); *device_ms = device time. */
+int hiqsim_batch_run(hiqsim_batch_t* b, double* device_ms);
+int hiqsim_batch_num_circuits(hiqsim_batch_t* b, int64_t* n);
+int hiqsim_batch_num_qubits(hiqsim_batch_t* b, int64_t index, int* n_qubits);
+/* Final state of circuit `index`: 2^n interleaved complex128 values. */
+int hiqsim_batch_get_state(hiqsim_batch_t* b, int64_t index, double* out);
+/* out[0..9] = circuits, fused passes, workgroup-owned circuits, LDS-resident
+ * circuits, layered launches, arena bytes, matrix bytes, amplitude-passes
+ * (sum over passes of 2^n), persistent workgroups, widest fusion width. */
+int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len);
+
+/* ---- request loop (continuous batching) ---------------------------------
+ * No counterpart in the reference.  Requests are circuits from |0...0> (as
+ * in a batch) with an output: kind 0 = `shots` samples (Philox seed, as
+ * hiqsim_sample), kind 1 = expectation values <P_t> of Pauli strings
+ * P_t = prod X^x_t Z^z_t with Y on bits set in both masks (bit i = qubit i).
+ * A request's state (2^n amplitudes) and fused matrices live in two device
+ * pools sized at creation; requests are admitted in arrival order as the
+ * pools allow and advance together, passes_per_step fused passes per step. */
+typedef struct hiqsim_server hiqsim_server_t;
+
+int hiqsim_server_create(int precision, int device, uint64_t state_bytes, uint64_t matrix_bytes,
+                         int passes_per_step, int max_active, hiqsim_server_t** out);
+int hiqsim_server_destroy(hiqsim_server_t* s);
+/* k_by_n[n] = fusion width (1..5, 0 = default 4) for requests of n qubits. */
+int hiqsim_server_set_widths(hiqsim_server_t* s, const int32_t* k_by_n, size_t len);
+/* Gates as in hiqsim_batch_add_circuit; masks hold nterms entries (kind 1). */
+int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
+                         const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,
+                         const size_t* ctrl_off, int kind, uint32_t shots, uint64_t seed,
+                         const uint64_t* xmask, const uint64_t* zmask, size_t nterms,
+                         int64_t* id);
+/* Collects finished work, admits and launches one step; with block != 0 it
+ * waits for in-flight work when nothing new can be launched.  The ids of
+ * the requests completed during the call go to completed[0..cap) and their
+ * number to *ncompleted (cap >= max_active never truncates). */
+int hiqsim_server_step(hiqsim_server_t* s, int block, int64_t* completed, size_t cap,
+                       size_t* ncompleted);
+int hiqsim_server_idle(hiqsim_server_t* s, int* idle);
+int hiqsim_server_done(hiqsim_server_t* s, int64_t id, int* done);
+/* Results of a completed request: samples (kind 0) or 2 * nterms doubles
+ * (Re, Im of each <P_t>, kind 1); *n = number of samples / terms. */
+int hiqsim_server_samples(hiqsim_server_t* s, int64_t id, uint64_t* out, size_t cap, size_t* n);
+int hiqsim_server_expectations(hiqsim_server_t* s, int64_t id, double* out, size_t cap_terms,
+                               size_t* n);
+/* out[0..2] = steady-clock seconds of submission, admission, completion. */
+int hiqsim_server_times(hiqsim_server_t* s, int64_t id, double* out);
+/* Drops a completed request's results. */
+int hiqsim_server_forget(hiqsim_server_t* s, int64_t id);
+/* out[0..5] = submitted, completed, steps, launches, peak active requests,
+ * peak state-pool bytes in use. */
+int hiqsim_server_stats(hiqsim_server_t* s, uint64_t* out, size_t len);
 
 #ifdef __cplusplus
 }

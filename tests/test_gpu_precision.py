@@ -7,7 +7,8 @@ complex128 and complex64, against FP64 references.
 * 32 qubits (64 GiB complex128 / 32 GiB complex64): QFT followed by its
   inverse must return the initial state (size-independent property),
   compared on the device in FP64.
-Bars: complex128 max-abs <= 1e-12; complex64 fidelity >= 1 - 1e-6.
+Bars: complex128 max-abs <= 1e-12; complex64 fidelity >= 1 - 1e-6, where
+fidelity = |<ref|psi>|^2 / (<ref|ref><psi|psi>).
 """
 import numpy as np
 import pytest
@@ -76,10 +77,15 @@ def test_qft_round_trip_32q(engine_mod, precision):
     apply_scheduled(g, gates, 4)
     apply_scheduled(g, inverse(gates), 4)
     r = g.compare_snapshot()
-    fid = abs(r["overlap"]) ** 2 / r["ref_norm"] ** 2
-    print("qft32 round trip %s: max-abs %.3e, 1 - fidelity %.3e"
-          % (precision, r["max_abs_diff"], 1 - fid))
+    fid = r["fidelity"]
+    norm_drift = abs(r["state_norm2"] / r["ref_norm2"] - 1.0)
+    print("qft32 round trip %s: max-abs %.3e, 1 - fidelity %.3e, norm drift %.3e"
+          % (precision, r["max_abs_diff"], 1 - fid, norm_drift))
     if precision == "c128":
         assert r["max_abs_diff"] <= 1e-12
+        assert 1 - fid <= 1e-12
     else:
+        # fidelity of the rays; the norm itself drifts by the complex64
+        # rounding of the fused matrices (about 1e-8 per pass)
         assert fid >= 1 - 1e-6
+        assert norm_drift <= 1e-5
