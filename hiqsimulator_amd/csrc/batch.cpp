@@ -113,10 +113,15 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 Batch::Batch(Prec prec, int device) : prec_(prec), device_(device)
 {
+     if (device < 0) throw std::runtime_error("Batch(): device must be >= 0");
+}
+
+void Batch::ensure_device()
+{
+     if (device_ready_) return;
      int ndev = 0;
      check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-     if (device < 0 || device >= ndev)
-          throw std::runtime_error("Batch(): no HIP device " + std::to_string(device));
+     if (device_ >= ndev) throw std::runtime_error("Batch(): no HIP device " + std::to_string(device_));
      DeviceScope ds(device_);
      check(hipStreamCreateWithFlags(&s1_, hipStreamNonBlocking), "hipStreamCreate");
      check(hipStreamCreateWithFlags(&s2_, hipStreamNonBlocking), "hipStreamCreate");
@@ -124,11 +129,14 @@ Batch::Batch(Prec prec, int device) : prec_(prec), device_(device)
      check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming), "hipEventCreate");
      check(hipEventCreate(&ev_t0_), "hipEventCreate");
      check(hipEventCreate(&ev_t1_), "hipEventCreate");
+     device_ready_ = true;
 }
 
 Batch::~Batch()
 {
+     if (!device_ready_) return;
      (void)hipSetDevice(device_);
+     (void)hipStreamSynchronize(s1_);
      release_device();
      (void)hipEventDestroy(ev_fork_);
      (void)hipEventDestroy(ev_join_);
@@ -211,11 +219,12 @@ int Batch::num_qubits(int64_t c) const
      return circ_[c].n;
 }
 
-void Batch::build()
+// Host plan: every circuit cut into clusters and fused, the arena laid out
+// (layered states first, one zero fill), the workgroup lists and the layered
+// launches.
+Batch::HostPlan Batch::plan_host(int workgroups)
 {
-     DeviceScope ds(device_);
-     check(hipStreamSynchronize(s1_), "hipStreamSynchronize");
-     release_device();
+     HostPlan hp;
      stats_ = BatchStats();
      const size_t nc = circ_.size();
      if (nc == 0) throw std::runtime_error("batch: no circuits");
@@ -224,8 +233,6 @@ void Batch::build()
      std::vector<std::vector<PlannedPass>> plan(nc);
      parallel_for(nc, [&](size_t c) { plan[c] = plan_circuit(circ_[c].gates, fusion_width(circ_[c].n)); });
 
-     // Classes and arena layout: layered states first (one zero fill), then
-     // the workgroup-owned ones.
      std::vector<size_t> layer_c, wg_c;
      for (size_t c = 0; c < nc; ++c) (circ_[c].n > wg_max_qubits_ ? layer_c : wg_c).push_back(c);
      uint64_t off = 0;
@@ -233,12 +240,12 @@ void Batch::build()
           circ_[c].state_off = off;
           off += uint64_t(1) << circ_[c].n;
      }
-     const uint64_t layer_amps = off;
+     hp.layer_amps = off;
      for (size_t c: wg_c) {
           circ_[c].state_off = off;
           off += uint64_t(1) << circ_[c].n;
      }
-     const uint64_t arena_amps = off;
+     hp.arena_amps = off;
 
      // Matrices, concatenated in the state's precision.
      std::vector<std::vector<uint64_t>> mat_off(nc);
@@ -253,7 +260,9 @@ void Batch::build()
           }
           stats_.passes += plan[c].size();
      }
-     std::vector<char> h_mats(std::max<uint64_t>(mat_entries, 1) * ab);
+     hp.mat_entries = mat_entries;
+     hp.mats.resize(std::max<uint64_t>(mat_entries, 1) * ab);
+     std::vector<char>& h_mats = hp.mats;
      parallel_for(nc, [&](size_t c) {
           for (size_t p = 0; p < plan[c].size(); ++p) {
                const auto& m = plan[c][p].m;
@@ -276,19 +285,13 @@ void Batch::build()
 
      // Workgroup-owned circuits: descriptors and a longest-first assignment
      // to workgroups by estimated cost (LDS-resident passes are cheaper).
-     int cus = 0;
-     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_),
-           "hipDeviceGetAttribute");
      const uint64_t lds_amps = kBatchLdsBytes / ab;
-     std::vector<BatchPass> h_pass;
-     std::vector<BatchCirc> h_circ;
      std::vector<double> cost;
-     int kmax_wg = 1;
      for (size_t c: wg_c) {
           BatchCirc bc{};
           bc.state_off = circ_[c].state_off;
           bc.n = static_cast<uint32_t>(circ_[c].n);
-          bc.pass_begin = static_cast<uint32_t>(h_pass.size());
+          bc.pass_begin = static_cast<uint32_t>(hp.pass.size());
           const uint64_t N = uint64_t(1) << circ_[c].n;
           const bool in_lds = N <= lds_amps;
           double cst = static_cast<double>(N);
@@ -297,18 +300,19 @@ void Batch::build()
                bp.mat_off = mat_off[c][p];
                bp.k = static_cast<uint8_t>(plan[c][p].k);
                std::memcpy(bp.pos, plan[c][p].pos, 5);
-               h_pass.push_back(bp);
-               kmax_wg = std::max(kmax_wg, plan[c][p].k);
+               hp.pass.push_back(bp);
+               hp.kmax_wg = std::max(hp.kmax_wg, plan[c][p].k);
                cst += static_cast<double>(N) * ((1 << plan[c][p].k) + (in_lds ? 2 : 8));
           }
-          bc.pass_end = static_cast<uint32_t>(h_pass.size());
-          h_circ.push_back(bc);
+          bc.pass_end = static_cast<uint32_t>(hp.pass.size());
+          hp.circ.push_back(bc);
           cost.push_back(cst);
           if (in_lds) ++stats_.lds_circuits;
      }
      stats_.wg_circuits = wg_c.size();
-     const int n_wg = static_cast<int>(std::min<size_t>(static_cast<size_t>(std::max(cus, 1)), wg_c.size()));
-     std::vector<uint32_t> h_wg_off(n_wg + 1, 0), h_wg_list;
+     hp.n_wg = static_cast<int>(std::min<size_t>(static_cast<size_t>(std::max(workgroups, 1)), wg_c.size()));
+     const int n_wg = hp.n_wg;
+     hp.wg_off.assign(n_wg + 1, 0);
      if (n_wg > 0) {
           std::vector<size_t> order(wg_c.size());
           for (size_t i = 0; i < order.size(); ++i) order[i] = i;
@@ -318,97 +322,229 @@ void Batch::build()
           for (int w = 0; w < n_wg; ++w) heap.push({0.0, w});
           std::vector<std::vector<uint32_t>> lists(n_wg);
           for (size_t i: order) {
-               Slot s = heap.top();
+               Slot sl = heap.top();
                heap.pop();
-               lists[s.second].push_back(static_cast<uint32_t>(i));
-               heap.push({s.first + cost[i], s.second});
+               lists[sl.second].push_back(static_cast<uint32_t>(i));
+               heap.push({sl.first + cost[i], sl.second});
           }
           for (int w = 0; w < n_wg; ++w) {
-               h_wg_off[w + 1] = h_wg_off[w] + static_cast<uint32_t>(lists[w].size());
-               h_wg_list.insert(h_wg_list.end(), lists[w].begin(), lists[w].end());
+               hp.wg_off[w + 1] = hp.wg_off[w] + static_cast<uint32_t>(lists[w].size());
+               hp.wg_list.insert(hp.wg_list.end(), lists[w].begin(), lists[w].end());
           }
      }
      stats_.wg_grid = n_wg;
-
-     // Layered circuits: launch (L, k) holds the L-th pass of every layered
-     // circuit whose L-th pass has width k.
-     struct LayerLaunch {
-          int k;
-          size_t first, count;
-          uint64_t blocks;
-     };
-     std::vector<BatchItem> h_items;
-     std::vector<LayerLaunch> launches;
+     // Layered circuits: launch (L, k, form) holds the L-th pass of every
+     // layered circuit whose L-th pass has width k and takes that form.  The
+     // wave-tile form is used unless every target is at bit >= 20 (chunks
+     // 2^20 amplitudes apart) or, for k = 5, bit 0 (bits 0..1 in complex64)
+     // is not a target (a 32 KiB-per-wave tile halves the occupancy).
+     std::vector<BatchItem>& h_items = hp.items;
+     std::vector<BatchTileItem>& h_titems = hp.titems;
+     std::vector<LayerLaunch>& launches = hp.launches;
      size_t max_len = 0;
      for (size_t c: layer_c) max_len = std::max(max_len, plan[c].size());
      for (size_t L = 0; L < max_len; ++L) {
           for (int k = 1; k <= kMaxWidth; ++k) {
-               LayerLaunch ll{k, h_items.size(), 0, 0};
+               LayerLaunch ld{k, false, h_items.size(), 0, 0};
+               LayerLaunch lt{k, true, h_titems.size(), 0, 0};
                for (size_t c: layer_c) {
                     if (L >= plan[c].size() || plan[c][L].k != k) continue;
+                    const PlannedPass& pp = plan[c][L];
+                    const int n = circ_[c].n;
+                    BatchTileItem ti{};
+                    const bool tile_pays = pp.pos[0] < 20 &&
+                                           (k < 5 || pp.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
+                    if (tile_pays && make_tile_item(n, k, pp.pos, prec_, ti)) {
+                         ti.state_off = circ_[c].state_off;
+                         ti.mat_off = mat_off[c][L];
+                         ti.block_start = lt.blocks;
+                         lt.blocks += (uint64_t(1) << (n - k)) / (64ull * batch_tile_wpb(k, prec_));
+                         h_titems.push_back(ti);
+                         hp.titem_n.push_back(n);
+                         ++lt.count;
+                         continue;
+                    }
                     BatchItem it{};
                     it.state_off = circ_[c].state_off;
                     it.mat_off = mat_off[c][L];
-                    it.block_start = ll.blocks;
-                    it.log_groups = static_cast<uint32_t>(circ_[c].n - k);
-                    std::memcpy(it.pos, plan[c][L].pos, 5);
+                    it.block_start = ld.blocks;
+                    it.log_groups = static_cast<uint32_t>(n - k);
+                    std::memcpy(it.pos, pp.pos, 5);
                     const uint64_t groups = uint64_t(1) << it.log_groups;
-                    ll.blocks += (groups + kBatchLayerBlock - 1) / kBatchLayerBlock;
+                    ld.blocks += (groups + kBatchLayerBlock - 1) / kBatchLayerBlock;
                     h_items.push_back(it);
-                    ++ll.count;
+                    hp.item_n.push_back(n);
+                    ++ld.count;
                }
-               if (ll.count) launches.push_back(ll);
+               if (ld.count) launches.push_back(ld);
+               if (lt.count) launches.push_back(lt);
           }
      }
      stats_.layer_launches = launches.size();
-     std::vector<uint64_t> h_ones;
-     for (size_t c: layer_c) h_ones.push_back(circ_[c].state_off);
-
-     // Device memory.
+     for (const auto& ll: launches)
+          if (ll.blocks > 65536 && ll.blocks % 65536 != 0) ++stats_.padded_launches;
+     for (size_t c: layer_c) hp.ones.push_back(circ_[c].state_off);
      stats_.circuits = nc;
-     stats_.arena_bytes = arena_amps * ab;
+     stats_.arena_bytes = hp.arena_amps * ab;
      stats_.matrix_bytes = mat_entries * ab;
+     return hp;
+}
+
+
+namespace {
+uint64_t insert_zero_bits(uint64_t t, const uint8_t* pos, int k)
+{
+     for (int l = 0; l < k; ++l) {
+          const uint64_t lo = t & ((uint64_t(1) << pos[l]) - 1);
+          t = ((t ^ lo) << 1) | lo;
+     }
+     return t;
+}
+[[noreturn]] void plan_error(const std::string& what)
+{
+     throw std::runtime_error("batch: internal plan check failed: " + what);
+}
+}  // namespace
+
+// Bounds of every launch against the buffers it touches: the kernels map
+// blocks to items through block_start, so the items of a launch must tile
+// [0, blocks) exactly and every address an item's blocks form must lie in
+// its state; matrix reads must lie in the matrix array.
+void Batch::check_plan(const HostPlan& hp) const
+{
+     auto state_ok = [&](uint64_t off, int n) { return n >= 1 && n <= 30 && off + (uint64_t(1) << n) <= hp.arena_amps; };
+     auto mat_ok = [&](uint64_t off, int k) { return k >= 1 && k <= 5 && off + (uint64_t(1) << (2 * k)) <= hp.mat_entries; };
+     auto pos_ok = [&](const uint8_t* pos, int k, int n) {
+          for (int l = 0; l < k; ++l)
+               if (pos[l] >= n || (l && pos[l] <= pos[l - 1])) return false;
+          return true;
+     };
+     // workgroup-owned circuits
+     if (hp.wg_off.size() != static_cast<size_t>(hp.n_wg) + 1 || hp.wg_off.back() != hp.wg_list.size() ||
+         hp.wg_list.size() != hp.circ.size())
+          plan_error("workgroup lists");
+     for (int w = 0; w < hp.n_wg; ++w)
+          if (hp.wg_off[w + 1] < hp.wg_off[w]) plan_error("workgroup offsets");
+     for (uint32_t ci: hp.wg_list)
+          if (ci >= hp.circ.size()) plan_error("workgroup list entry");
+     for (const BatchCirc& c: hp.circ) {
+          if (static_cast<int>(c.n) > kMaxWgQubits || !state_ok(c.state_off, static_cast<int>(c.n)))
+               plan_error("workgroup state");
+          if (c.pass_end < c.pass_begin || c.pass_end > hp.pass.size()) plan_error("workgroup passes");
+          for (uint32_t p = c.pass_begin; p < c.pass_end; ++p) {
+               const BatchPass& bp = hp.pass[p];
+               if (bp.k > hp.kmax_wg || bp.k > static_cast<int>(c.n) || !mat_ok(bp.mat_off, bp.k) ||
+                   !pos_ok(bp.pos, bp.k, static_cast<int>(c.n)))
+                    plan_error("workgroup pass");
+               }
+     }
+     if (hp.kmax_wg < 1 || hp.kmax_wg > 5) plan_error("workgroup width");
+     // layered launches
+     for (const LayerLaunch& ll: hp.launches) {
+          uint64_t next = 0;
+          for (size_t i = ll.first; i < ll.first + ll.count; ++i) {
+               if (ll.tile) {
+                    if (i >= hp.titems.size()) plan_error("tile item index");
+                    const BatchTileItem& it = hp.titems[i];
+                    const int n = hp.titem_n[i];
+                    const int wpb = batch_tile_wpb(ll.k, prec_);
+                    const uint64_t nb = (uint64_t(1) << (n - ll.k)) / (64ull * wpb);
+                    if (it.block_start != next || nb == 0 || nb * 64ull * wpb != (uint64_t(1) << (n - ll.k)))
+                         plan_error("tile item blocks");
+                    next += nb;
+                    if (it.nins != ll.k || !pos_ok(it.ins, ll.k, n) || !state_ok(it.state_off, n) ||
+                        !mat_ok(it.mat_off, ll.k))
+                         plan_error("tile item");
+                    const uint64_t last = insert_zero_bits((nb * wpb - 1) << 6, it.ins, ll.k) + 63 +
+                                          it.hoff[(1 << ll.k) - 1];
+                    if (last >= (uint64_t(1) << n)) plan_error("tile item address");
+               }
+               else {
+                    if (i >= hp.items.size()) plan_error("item index");
+                    const BatchItem& it = hp.items[i];
+                    const int n = hp.item_n[i];
+                    if (it.log_groups != static_cast<uint32_t>(n - ll.k)) plan_error("item groups");
+                    const uint64_t groups = uint64_t(1) << it.log_groups;
+                    const uint64_t nb = (groups + kBatchLayerBlock - 1) / kBatchLayerBlock;
+                    if (it.block_start != next) plan_error("item blocks");
+                    next += nb;
+                    if (!pos_ok(it.pos, ll.k, n) || !state_ok(it.state_off, n) || !mat_ok(it.mat_off, ll.k))
+                         plan_error("item");
+                    uint64_t span = 0;
+                    for (int l = 0; l < ll.k; ++l) span |= uint64_t(1) << it.pos[l];
+                    if (insert_zero_bits(groups - 1, it.pos, ll.k) + span >= (uint64_t(1) << n))
+                         plan_error("item address");
+               }
+          }
+          if (next != ll.blocks) plan_error("launch blocks");
+     }
+     for (uint64_t o: hp.ones)
+          if (o >= hp.layer_amps) plan_error("initial amplitude offset");
+}
+
+void Batch::prepare()
+{
+     HostPlan hp = plan_host(kDefaultWorkgroups);
+     check_plan(hp);
+}
+
+void Batch::build()
+{
+     ensure_device();
+     DeviceScope ds(device_);
+     check(hipStreamSynchronize(s1_), "hipStreamSynchronize");
+     release_device();
+     int cus = 0;
+     check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_),
+           "hipDeviceGetAttribute");
+     HostPlan hp = plan_host(cus);
+     check_plan(hp);
+     const int n_wg = hp.n_wg;
+     const bool has_layer = hp.layer_amps > 0;
+     // Device memory.
      size_t free_b = 0, total_b = 0;
      check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-     const size_t sz_pass = align_up(h_pass.size() * sizeof(BatchPass), 256);
-     const size_t sz_circ = align_up(h_circ.size() * sizeof(BatchCirc), 256);
-     const size_t sz_woff = align_up(h_wg_off.size() * sizeof(uint32_t), 256);
-     const size_t sz_wl = align_up(h_wg_list.size() * sizeof(uint32_t), 256);
-     const size_t sz_items = align_up(h_items.size() * sizeof(BatchItem), 256);
-     const size_t sz_ones = align_up(h_ones.size() * sizeof(uint64_t), 256);
-     const size_t desc_bytes = sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_ones + 256;
-     const size_t need = stats_.arena_bytes + h_mats.size() + desc_bytes;
+     const size_t sz_pass = align_up(hp.pass.size() * sizeof(BatchPass), 256);
+     const size_t sz_circ = align_up(hp.circ.size() * sizeof(BatchCirc), 256);
+     const size_t sz_woff = align_up(hp.wg_off.size() * sizeof(uint32_t), 256);
+     const size_t sz_wl = align_up(hp.wg_list.size() * sizeof(uint32_t), 256);
+     const size_t sz_items = align_up(hp.items.size() * sizeof(BatchItem), 256);
+     const size_t sz_titems = align_up(hp.titems.size() * sizeof(BatchTileItem), 256);
+     const size_t sz_ones = align_up(hp.ones.size() * sizeof(uint64_t), 256);
+     const size_t desc_bytes = sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_titems + sz_ones + 256;
+     const size_t need = stats_.arena_bytes + hp.mats.size() + desc_bytes;
      if (need + (size_t(1) << 30) > free_b) {
           std::ostringstream os;
           os << "batch: needs " << need << " bytes of device memory, " << free_b << " free";
           throw std::runtime_error(os.str());
      }
      check(hipMalloc(&d_arena_, std::max<size_t>(stats_.arena_bytes, 256)), "hipMalloc(arena)");
-     check(hipMalloc(&d_mats_, h_mats.size()), "hipMalloc(matrices)");
+     check(hipMalloc(&d_mats_, hp.mats.size()), "hipMalloc(matrices)");
      check(hipMalloc(&d_desc_, desc_bytes), "hipMalloc(descriptors)");
-     check(hipMemcpy(d_mats_, h_mats.data(), h_mats.size(), hipMemcpyHostToDevice), "hipMemcpy(matrices)");
+     check(hipMemcpy(d_mats_, hp.mats.data(), hp.mats.size(), hipMemcpyHostToDevice), "hipMemcpy(matrices)");
      char* base = static_cast<char*>(d_desc_);
      auto* d_pass = reinterpret_cast<BatchPass*>(base);
      auto* d_circ = reinterpret_cast<BatchCirc*>(base + sz_pass);
      auto* d_woff = reinterpret_cast<uint32_t*>(base + sz_pass + sz_circ);
      auto* d_wl = reinterpret_cast<uint32_t*>(base + sz_pass + sz_circ + sz_woff);
      auto* d_items = reinterpret_cast<BatchItem*>(base + sz_pass + sz_circ + sz_woff + sz_wl);
-     auto* d_ones = reinterpret_cast<uint64_t*>(base + sz_pass + sz_circ + sz_woff + sz_wl + sz_items);
+     auto* d_titems = reinterpret_cast<BatchTileItem*>(base + sz_pass + sz_circ + sz_woff + sz_wl + sz_items);
+     auto* d_ones = reinterpret_cast<uint64_t*>(base + sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_titems);
      auto up = [](void* d, const void* h, size_t bytes) {
           if (bytes) check(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "hipMemcpy(descriptors)");
      };
-     up(d_pass, h_pass.data(), h_pass.size() * sizeof(BatchPass));
-     up(d_circ, h_circ.data(), h_circ.size() * sizeof(BatchCirc));
-     up(d_woff, h_wg_off.data(), h_wg_off.size() * sizeof(uint32_t));
-     up(d_wl, h_wg_list.data(), h_wg_list.size() * sizeof(uint32_t));
-     up(d_items, h_items.data(), h_items.size() * sizeof(BatchItem));
-     up(d_ones, h_ones.data(), h_ones.size() * sizeof(uint64_t));
-     plan.clear();
+     up(d_pass, hp.pass.data(), hp.pass.size() * sizeof(BatchPass));
+     up(d_circ, hp.circ.data(), hp.circ.size() * sizeof(BatchCirc));
+     up(d_woff, hp.wg_off.data(), hp.wg_off.size() * sizeof(uint32_t));
+     up(d_wl, hp.wg_list.data(), hp.wg_list.size() * sizeof(uint32_t));
+     up(d_items, hp.items.data(), hp.items.size() * sizeof(BatchItem));
+     up(d_titems, hp.titems.data(), hp.titems.size() * sizeof(BatchTileItem));
+     up(d_ones, hp.ones.data(), hp.ones.size() * sizeof(uint64_t));
 
      // Capture: the persistent launch runs on a forked stream beside the
      // layered chain; both join before the graph ends.
      check(hipStreamBeginCapture(s1_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-     const bool fork = n_wg > 0 && !layer_c.empty();
+     const bool fork = n_wg > 0 && has_layer;
      hipStream_t ws = fork ? s2_ : s1_;
      if (fork) {
           check(hipEventRecord(ev_fork_, s1_), "hipEventRecord");
@@ -416,17 +552,23 @@ void Batch::build()
      }
      if (n_wg > 0)
           check(launch_batch_wg(d_arena_, prec_, d_mats_, d_pass, d_circ, d_woff, d_wl, n_wg,
-                                kmax_wg, ws),
+                                hp.kmax_wg, ws),
                 "launch_batch_wg");
      if (fork) check(hipEventRecord(ev_join_, s2_), "hipEventRecord");
-     if (!layer_c.empty()) {
-          check(launch_fill(d_arena_, prec_, layer_amps, 0.0, 0.0, s1_), "launch_fill");
-          check(launch_set_ones(d_arena_, prec_, d_ones, static_cast<int>(h_ones.size()), s1_),
+     if (has_layer) {
+          check(launch_fill(d_arena_, prec_, hp.layer_amps, 0.0, 0.0, s1_), "launch_fill");
+          check(launch_set_ones(d_arena_, prec_, d_ones, static_cast<int>(hp.ones.size()), s1_),
                 "launch_set_ones");
-          for (const auto& ll: launches)
-               check(launch_batch_layer(d_arena_, prec_, d_mats_, d_items + ll.first,
-                                        static_cast<int>(ll.count), ll.k, ll.blocks, s1_),
-                     "launch_batch_layer");
+          for (const auto& ll: hp.launches) {
+               if (ll.tile)
+                    check(launch_batch_tile(d_arena_, prec_, d_mats_, d_titems + ll.first,
+                                            static_cast<int>(ll.count), ll.k, ll.blocks, s1_),
+                          "launch_batch_tile");
+               else
+                    check(launch_batch_layer(d_arena_, prec_, d_mats_, d_items + ll.first,
+                                             static_cast<int>(ll.count), ll.k, ll.blocks, s1_),
+                          "launch_batch_layer");
+          }
      }
      if (fork) check(hipStreamWaitEvent(s1_, ev_join_, 0), "hipStreamWaitEvent");
      check(hipStreamEndCapture(s1_, &graph_), "hipStreamEndCapture");

@@ -58,6 +58,32 @@ struct BatchItem {
      uint8_t pad[2];
 };
 
+// One pass of one large circuit in the wave-tile form: each wave owns 64
+// groups whose 64 * 2^k amplitudes are 2^k contiguous 64-amplitude chunks
+// at base + lane + hoff[i]; they are regrouped through LDS (slot r of the
+// tile = index bits at the tile positions, rank_n / rank_t give the ranks
+// of the lane bits and matrix bits).  Items of a launch share k.
+struct BatchTileItem {
+     uint64_t state_off;
+     uint64_t mat_off;
+     uint64_t block_start;
+     uint32_t hoff[32];
+     uint8_t ins[5];  // the pass's positions, ascending
+     uint8_t nins;
+     uint8_t rank_n[6];
+     uint8_t rank_t[5];
+     uint8_t pad[7];
+};
+
+// Waves per block of the batched tile kernel (LDS holds 64 * 2^k amplitudes
+// per wave).
+inline int batch_tile_wpb(int k, Prec prec) { return (64 << k) * static_cast<int>(amp_bytes(prec)) > 16384 ? 2 : 4; }
+// Builds the tile item of a k-qubit pass at ascending positions pos over a
+// state of 2^n amplitudes; false if the tile form does not apply.
+bool make_tile_item(int n, int k, const uint8_t* pos, Prec prec, BatchTileItem& it);
+hipError_t launch_batch_tile(void* arena, Prec prec, const void* mats, const BatchTileItem* items,
+                             int nitems, int k, uint64_t total_blocks, hipStream_t s);
+
 constexpr int kBatchLayerBlock = 256;
 // A state of at most this many bytes stays in LDS for its whole circuit.
 constexpr size_t kBatchLdsBytes = 128 * 1024;
@@ -137,6 +163,7 @@ struct BatchStats {
      uint64_t amp_passes = 0;       // sum over passes of 2^n (amplitudes touched)
      int wg_grid = 0;
      int kmax = 0;                  // widest fused pass
+     uint64_t padded_launches = 0;  // layered launches whose 2-D grid has padding blocks
 };
 
 class Batch {
@@ -154,7 +181,11 @@ public:
      // k_by_n[n] = fusion width for circuits of n qubits (0 = default 4);
      // circuits of more than wg_max_qubits qubits take the layered path.
      void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits);
-     // Plans, fuses, uploads and captures the graph (replaces a previous build).
+     // Plans and fuses every circuit and lays out the device descriptors on
+     // the host, then checks every launch's blocks and addresses against the
+     // arena, matrix and item bounds (throws on a violation).  Needs no device.
+     void prepare();
+     // prepare(), then uploads and captures the graph (replaces a previous build).
      void build();
      // Replays the graph and waits; returns the device time in ms.
      double run();
@@ -164,13 +195,40 @@ public:
      const BatchStats& stats() const { return stats_; }
      int fusion_width(int n) const;
 
+     // Persistent workgroups used when no device is queried (prepare()).
+     static constexpr int kDefaultWorkgroups = 256;
+
 private:
      struct Circ {
           int n;
           std::vector<BatchGate> gates;
           uint64_t state_off = 0;
      };
+     struct LayerLaunch {
+          int k;
+          bool tile;
+          size_t first, count;
+          uint64_t blocks;
+     };
+     // Host image of a build: everything the device needs.
+     struct HostPlan {
+          std::vector<char> mats;
+          uint64_t mat_entries = 0;
+          uint64_t arena_amps = 0, layer_amps = 0;
+          std::vector<BatchPass> pass;
+          std::vector<BatchCirc> circ;
+          std::vector<uint32_t> wg_off, wg_list;
+          int n_wg = 0, kmax_wg = 1;
+          std::vector<BatchItem> items;
+          std::vector<BatchTileItem> titems;
+          std::vector<int> item_n, titem_n;  // qubits of each item's circuit (checks)
+          std::vector<LayerLaunch> launches;
+          std::vector<uint64_t> ones;
+     };
+     void ensure_device();
      void release_device();
+     HostPlan plan_host(int workgroups);
+     void check_plan(const HostPlan& hp) const;
 
      Prec prec_;
      int device_;
@@ -182,6 +240,7 @@ private:
      std::vector<int> k_by_n_;
      int wg_max_qubits_ = 16;
      bool built_ = false;
+     bool device_ready_ = false;
      void* d_arena_ = nullptr;
      void* d_mats_ = nullptr;
      void* d_desc_ = nullptr;  // passes, circs, wg lists, items, one-offsets

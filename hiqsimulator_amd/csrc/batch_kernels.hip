@@ -143,11 +143,12 @@ __global__ void __launch_bounds__(NT)
 template <int K, typename R>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_batch_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
-                  const BatchItem* __restrict__ items, const int nitems)
+                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
 {
      using V = VT<R>;
      constexpr int D = 1 << K;
      const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid
      int lo = 0, hi = nitems - 1;
      while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -181,7 +182,8 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
 
 // Item lookup of a block-mapped launch: the last item whose first block is
 // at or below b.
-__device__ __forceinline__ int find_item(const BatchItem* __restrict__ items, int nitems, uint64_t b)
+template <typename Item>
+__device__ __forceinline__ int find_item(const Item* __restrict__ items, int nitems, uint64_t b)
 {
      int lo = 0, hi = nitems - 1;
      while (lo < hi) {
@@ -223,9 +225,10 @@ __device__ __forceinline__ void layer_group(VT<R>* __restrict__ arena, const VT<
 template <typename R, int KMAX>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_serve_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
-                  const BatchItem* __restrict__ items, const int nitems)
+                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
 {
      const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid
      const BatchItem it = items[find_item(items, nitems, b)];
      const uint64_t t = (b - it.block_start) * kBatchLayerBlock + threadIdx.x;
      if ((t >> it.log_groups) != 0) return;
@@ -240,13 +243,64 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
      }
 }
 
+// Batched wave-tile pass (the single-state k_dense_tile applied to the item
+// the block belongs to).
+template <int K, typename R, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
+    k_batch_tile(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+                 const BatchTileItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
+{
+     using V = VT<R>;
+     constexpr int D = 1 << K;
+     __shared__ V tile[WPB][64 * D];
+     const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
+     const BatchTileItem& it = items[find_item(items, nitems, b)];
+     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+     V* const T = tile[wv];
+     uint64_t idx = ((b - it.block_start) * WPB + wv) << 6;
+#pragma unroll
+     for (int l = 0; l < K; ++l) {
+          const uint64_t lo_bits = idx & ((1ull << it.ins[l]) - 1ull);
+          idx = ((idx ^ lo_bits) << 1) | lo_bits;
+     }
+     V* const base = arena + it.state_off + idx + lane;
+     V v[D];
+#pragma unroll
+     for (int i = 0; i < D; ++i) v[i] = ld<true>(base + it.hoff[i]);
+#pragma unroll
+     for (int i = 0; i < D; ++i) T[tile_slot<V>(i * 64 + lane)] = v[i];
+     __syncthreads();
+     int rl = 0;
+#pragma unroll
+     for (int q = 0; q < 6; ++q) rl |= ((lane >> q) & 1) << it.rank_n[q];
+     int slot[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          int r = rl;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) r |= 1 << it.rank_t[l];
+          slot[j] = tile_slot<V>(r);
+     }
+#pragma unroll
+     for (int j = 0; j < D; ++j) v[j] = T[slot[j]];
+     __syncthreads();
+     group_matvec<K, V, R>(mats + it.mat_off, v, [&](int i, V o) { T[slot[i]] = o; });
+     __syncthreads();
+#pragma unroll
+     for (int i = 0; i < D; ++i) st<true>(base + it.hoff[i], T[tile_slot<V>(i * 64 + lane)]);
+}
+
 template <typename R>
 __global__ void __launch_bounds__(kBatchLayerBlock)
-    k_init_states(VT<R>* __restrict__ arena, const BatchItem* __restrict__ items, const int nitems)
+    k_init_states(VT<R>* __restrict__ arena, const BatchItem* __restrict__ items, const int nitems,
+                  const uint64_t total_blocks)
 {
      using V = VT<R>;
      constexpr int E = kInitAmpsPerBlock / kBatchLayerBlock;
      const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid
      const BatchItem it = items[find_item(items, nitems, b)];
      const uint64_t first = (b - it.block_start) * kInitAmpsPerBlock;
      const uint64_t N = 1ull << it.log_groups;
@@ -260,9 +314,10 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
 template <typename R>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_block_sums_multi(const VT<R>* __restrict__ arena, const BatchItem* __restrict__ items,
-                       const int nitems, double* __restrict__ out)
+                       const int nitems, const uint64_t total_blocks, double* __restrict__ out)
 {
      const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
      const BatchItem it = items[find_item(items, nitems, b)];
      const uint64_t j = b - it.block_start;
      const uint64_t K = 1ull << it.log_groups;
@@ -396,11 +451,11 @@ hipError_t launch_batch_layer(void* arena, Prec prec, const void* mats, const Ba
           if (prec == Prec::C128)                                                                 \
                hipLaunchKernelGGL((k_batch_layer<KK, double>), grid, dim3(kBatchLayerBlock), 0,   \
                                   s, static_cast<double2*>(arena),                                \
-                                  static_cast<const double2*>(mats), items, nitems);              \
+                                  static_cast<const double2*>(mats), items, nitems, total_blocks); \
           else                                                                                    \
                hipLaunchKernelGGL((k_batch_layer<KK, float>), grid, dim3(kBatchLayerBlock), 0, s, \
                                   static_cast<float2*>(arena), static_cast<const float2*>(mats),  \
-                                  items, nitems);                                                 \
+                                  items, nitems, total_blocks);                                   \
      } while (0)
      switch (k) {
           case 1: HQ_LAYER(1); break;
@@ -423,7 +478,7 @@ hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const Ba
 #define HQ_SERVE(RR, KM)                                                                         \
      hipLaunchKernelGGL((k_serve_layer<RR, KM>), grid, dim3(kBatchLayerBlock), 0, s,              \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
-                        nitems)
+                        nitems, total_blocks)
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_SERVE(double, 5);
           else HQ_SERVE(double, 4);
@@ -436,6 +491,79 @@ hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const Ba
      return hipGetLastError();
 }
 
+bool make_tile_item(int n, int k, const uint8_t* pos, Prec prec, BatchTileItem& it)
+{
+     const int wpb = batch_tile_wpb(k, prec);
+     if (k < 1 || k > 5 || n - k < 6 + (wpb == 4 ? 2 : 1)) return false;
+     uint64_t tmask = 0;
+     for (int l = 0; l < k; ++l) tmask |= 1ull << pos[l];
+     // lanes: the six lowest non-target positions
+     uint64_t lanes = 0;
+     int nb = 0;
+     for (int p = 0; p < n && nb < 6; ++p)
+          if (!((tmask >> p) & 1)) {
+               lanes |= 1ull << p;
+               ++nb;
+          }
+     const uint64_t vbits = lanes | tmask;
+     if ((vbits & 63) != 63) return false;
+     int vpos[64], rank_of[64], nv = 0;
+     for (int p = 0; p < 64; ++p)
+          if ((vbits >> p) & 1) {
+               rank_of[p] = nv;
+               vpos[nv++] = p;
+          }
+     if (nv != 6 + k) return false;
+     nb = 0;
+     for (int p = 0; p < 64; ++p)
+          if ((lanes >> p) & 1) it.rank_n[nb++] = static_cast<uint8_t>(rank_of[p]);
+     for (int l = 0; l < k; ++l) {
+          it.rank_t[l] = static_cast<uint8_t>(rank_of[pos[l]]);
+          it.ins[l] = pos[l];
+     }
+     it.nins = static_cast<uint8_t>(k);
+     for (int i = 0; i < 32; ++i) {
+          uint64_t o = 0;
+          if (i < (1 << k))
+               for (int q = 0; q < k; ++q)
+                    if ((i >> q) & 1) o |= 1ull << vpos[6 + q];
+          it.hoff[i] = static_cast<uint32_t>(o);
+     }
+     return true;
+}
+
+hipError_t launch_batch_tile(void* arena, Prec prec, const void* mats, const BatchTileItem* items,
+                             int nitems, int k, uint64_t total_blocks, hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     const dim3 grid = grid_2d(total_blocks);
+#define HQ_TILE(KK)                                                                              \
+     do {                                                                                         \
+          if (prec == Prec::C128) {                                                               \
+               constexpr int W = (64 << KK) * 16 > 16384 ? 2 : 4;                                 \
+               hipLaunchKernelGGL((k_batch_tile<KK, double, W>), grid, dim3(64 * W), 0, s,        \
+                                  static_cast<double2*>(arena), static_cast<const double2*>(mats), \
+                                  items, nitems, total_blocks);                                   \
+          }                                                                                       \
+          else {                                                                                  \
+               constexpr int W = (64 << KK) * 8 > 16384 ? 2 : 4;                                  \
+               hipLaunchKernelGGL((k_batch_tile<KK, float, W>), grid, dim3(64 * W), 0, s,         \
+                                  static_cast<float2*>(arena), static_cast<const float2*>(mats),   \
+                                  items, nitems, total_blocks);                                   \
+          }                                                                                       \
+     } while (0)
+     switch (k) {
+          case 1: HQ_TILE(1); break;
+          case 2: HQ_TILE(2); break;
+          case 3: HQ_TILE(3); break;
+          case 4: HQ_TILE(4); break;
+          case 5: HQ_TILE(5); break;
+          default: return hipErrorInvalidValue;
+     }
+#undef HQ_TILE
+     return hipGetLastError();
+}
+
 hipError_t launch_init_states(void* arena, Prec prec, const BatchItem* items, int nitems,
                               uint64_t total_blocks, hipStream_t s)
 {
@@ -443,10 +571,10 @@ hipError_t launch_init_states(void* arena, Prec prec, const BatchItem* items, in
      const dim3 grid = grid_2d(total_blocks);
      if (prec == Prec::C128)
           hipLaunchKernelGGL((k_init_states<double>), grid, dim3(kBatchLayerBlock), 0, s,
-                             static_cast<double2*>(arena), items, nitems);
+                             static_cast<double2*>(arena), items, nitems, total_blocks);
      else
           hipLaunchKernelGGL((k_init_states<float>), grid, dim3(kBatchLayerBlock), 0, s,
-                             static_cast<float2*>(arena), items, nitems);
+                             static_cast<float2*>(arena), items, nitems, total_blocks);
      return hipGetLastError();
 }
 
@@ -457,10 +585,10 @@ hipError_t launch_block_sums_multi(const void* arena, Prec prec, const BatchItem
      const dim3 grid = grid_2d(total_blocks);
      if (prec == Prec::C128)
           hipLaunchKernelGGL((k_block_sums_multi<double>), grid, dim3(kBatchLayerBlock), 0, s,
-                             static_cast<const double2*>(arena), items, nitems, out);
+                             static_cast<const double2*>(arena), items, nitems, total_blocks, out);
      else
           hipLaunchKernelGGL((k_block_sums_multi<float>), grid, dim3(kBatchLayerBlock), 0, s,
-                             static_cast<const float2*>(arena), items, nitems, out);
+                             static_cast<const float2*>(arena), items, nitems, total_blocks, out);
      return hipGetLastError();
 }
 

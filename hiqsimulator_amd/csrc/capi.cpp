@@ -529,6 +529,14 @@ int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, 
      });
 }
 
+int hiqsim_batch_prepare(hiqsim_batch_t* b)
+{
+     return guard([&] {
+          need(b, "batch");
+          b->b.prepare();
+     });
+}
+
 int hiqsim_batch_build(hiqsim_batch_t* b)
 {
      return guard([&] {
@@ -579,11 +587,11 @@ int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len)
           need(b, "batch");
           if (len) need(out, "out");
           const hq::BatchStats& s = b->b.stats();
-          const uint64_t v[10] = {s.circuits,       s.passes,      s.wg_circuits,
+          const uint64_t v[11] = {s.circuits,       s.passes,      s.wg_circuits,
                                   s.lds_circuits,   s.layer_launches, s.arena_bytes,
                                   s.matrix_bytes,   s.amp_passes,  static_cast<uint64_t>(s.wg_grid),
-                                  static_cast<uint64_t>(s.kmax)};
-          for (size_t i = 0; i < len && i < 10; ++i) out[i] = v[i];
+                                  static_cast<uint64_t>(s.kmax), s.padded_launches};
+          for (size_t i = 0; i < len && i < 11; ++i) out[i] = v[i];
      });
 }
 

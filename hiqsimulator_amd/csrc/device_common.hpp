@@ -140,6 +140,16 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
      return c;
 }
 
+// LDS slot of tile element r: XOR-swizzled so that both the chunk-order
+// accesses (consecutive r across lanes) and the group-order accesses (lanes
+// strided by powers of two) spread over the banks.
+template <typename V>
+__device__ __forceinline__ int tile_slot(int r)
+{
+     constexpr int S = sizeof(V) == 16 ? 4 : 5;
+     return r ^ ((r >> S) & ((1 << S) - 1));
+}
+
 // 53-bit uniform in [0, 1) from two 32-bit Philox outputs.
 __device__ __forceinline__ double uniform53(uint32_t hi, uint32_t lo)
 {

@@ -79,16 +79,6 @@ struct TileDesc {
      uint8_t rank_t[5];   // tile rank of matrix bit l
 };
 
-// LDS slot of tile element r: XOR-swizzled so that both the chunk-order
-// accesses (consecutive r across lanes) and the group-order accesses (lanes
-// strided by powers of two) spread over the banks.
-template <typename V>
-__device__ __forceinline__ int tile_slot(int r)
-{
-     constexpr int S = sizeof(V) == 16 ? 4 : 5;
-     return r ^ ((r >> S) & ((1 << S) - 1));
-}
-
 template <int K, typename R, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
     k_dense_tile(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,

@@ -497,6 +497,11 @@ public:
           }
           ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits));
      }
+     void prepare()
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_batch_prepare(h_));
+     }
      void build()
      {
           py::gil_scoped_release r;
@@ -534,14 +539,14 @@ public:
      }
      py::dict stats() const
      {
-          uint64_t v[10] = {0};
-          ok(hiqsim_batch_stats(h_, v, 10));
-          static const char* names[10] = {"circuits",      "passes",        "wg_circuits",
+          uint64_t v[11] = {0};
+          ok(hiqsim_batch_stats(h_, v, 11));
+          static const char* names[11] = {"circuits",      "passes",         "wg_circuits",
                                           "lds_circuits",  "layer_launches", "arena_bytes",
-                                          "matrix_bytes",  "amp_passes",    "wg_grid",
-                                          "kmax"};
+                                          "matrix_bytes",  "amp_passes",     "wg_grid",
+                                          "kmax",          "padded_launches"};
           py::dict d;
-          for (int i = 0; i < 10; ++i) d[names[i]] = v[i];
+          for (int i = 0; i < 11; ++i) d[names[i]] = v[i];
           return d;
      }
 
@@ -744,6 +749,7 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("add_circuit", &PyBatch::add_circuit, py::arg("n_qubits"), py::arg("gates"))
          .def("add_copies", &PyBatch::add_copies, py::arg("index"), py::arg("count"))
          .def("set_plan", &PyBatch::set_plan, py::arg("k_by_n"), py::arg("wg_max_qubits") = 16)
+         .def("prepare", &PyBatch::prepare)
          .def("build", &PyBatch::build)
          .def("run", &PyBatch::run)
          .def("__len__", &PyBatch::size)

@@ -2,6 +2,7 @@
 #include "serve.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
 #include <sstream>
@@ -38,15 +39,25 @@ double now_s()
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Blocks of one item array laid out back to back (block_start ascending).
-struct ItemList {
-     std::vector<BatchItem> items;
+template <typename Item>
+struct ItemListT {
+     std::vector<Item> items;
      uint64_t blocks = 0;
-     void add(BatchItem it, uint64_t nblocks)
+     void add(Item it, uint64_t nblocks)
      {
           it.block_start = blocks;
           blocks += nblocks;
           items.push_back(it);
      }
+};
+using ItemList = ItemListT<BatchItem>;
+using TileList = ItemListT<BatchTileItem>;
+
+// Pass l of a step: the wave-tile items by width and the direct-form rest
+// (widths mixed).
+struct StepLayer {
+     ItemList direct;
+     TileList tile[6];
 };
 
 }  // namespace
@@ -124,6 +135,7 @@ Server::Server(Prec prec, int device, uint64_t state_bytes, uint64_t matrix_byte
            "hipMalloc(scratch)");
      state_pool_ = RangeAlloc(state_cap_amps_);
      mat_pool_ = RangeAlloc(mat_entries);
+     mat_cap_entries_ = mat_entries;
      for (auto& s: slots_) check(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
 }
 
@@ -369,21 +381,45 @@ std::vector<int64_t> Server::step(bool block)
           it.log_groups = static_cast<uint32_t>(r.n);
           init.add(it, ((uint64_t(1) << r.n) + kInitAmpsPerBlock - 1) / kInitAmpsPerBlock);
      }
-     std::vector<ItemList> layers(pps_);
+     // The wave-tile form is used under the same rule as in a batch (not
+     // when every target is at bit >= 20, nor for k = 5 without bit 0 (0..1
+     // in complex64) among the targets, nor for states too small for a
+     // block of tiles).
+     std::vector<StepLayer> layers(pps_);
      int kmax = 1;
      std::vector<int64_t> finishing;
      for (int64_t id: active_) {
           Req& r = reqs_.at(id);
           for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
                const Pass& p = r.passes[r.next + l];
+               const uint64_t moff = r.mat_off + r.pass_mat[r.next + l];
+               const bool tile_pays = p.pos[0] < 20 && (p.k < 5 || p.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
+               if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ ||
+                   moff + (uint64_t(1) << (2 * p.k)) > mat_cap_entries_ || p.k < 1 || p.k > r.n)
+                    throw std::runtime_error("Server: internal bounds check failed");
+               BatchTileItem ti{};
+               if (tile_pays && make_tile_item(r.n, p.k, p.pos, prec_, ti)) {
+                    ti.state_off = r.state_off;
+                    ti.mat_off = moff;
+                    const uint64_t nb = (uint64_t(1) << (r.n - p.k)) / (64ull * batch_tile_wpb(p.k, prec_));
+                    uint64_t last = (nb * batch_tile_wpb(p.k, prec_) - 1) << 6;
+                    for (int q = 0; q < p.k; ++q) {
+                         const uint64_t lo = last & ((uint64_t(1) << p.pos[q]) - 1);
+                         last = ((last ^ lo) << 1) | lo;
+                    }
+                    if (nb == 0 || last + 63 + ti.hoff[(1 << p.k) - 1] >= (uint64_t(1) << r.n))
+                         throw std::runtime_error("Server: internal tile bounds check failed");
+                    layers[l].tile[p.k].add(ti, nb);
+                    continue;
+               }
                BatchItem it{};
                it.state_off = r.state_off;
-               it.mat_off = r.mat_off + r.pass_mat[r.next + l];
+               it.mat_off = moff;
                it.log_groups = static_cast<uint32_t>(r.n - p.k);
                std::memcpy(it.pos, p.pos, 5);
                it.k = static_cast<uint8_t>(p.k);
                kmax = std::max(kmax, p.k);
-               layers[l].add(it, ((uint64_t(1) << it.log_groups) + kBatchLayerBlock - 1) / kBatchLayerBlock);
+               layers[l].direct.add(it, ((uint64_t(1) << it.log_groups) + kBatchLayerBlock - 1) / kBatchLayerBlock);
           }
           if (r.next + pps_ >= r.passes.size()) finishing.push_back(id);
      }
@@ -429,9 +465,14 @@ std::vector<int64_t> Server::step(bool block)
      const size_t o_init = 0;
      size_t o = align_up(init.items.size() * sizeof(BatchItem), 256);
      std::vector<size_t> o_layer(pps_);
+     std::vector<std::array<size_t, 6>> o_tile(pps_);
      for (int l = 0; l < pps_; ++l) {
           o_layer[l] = o;
-          o += align_up(layers[l].items.size() * sizeof(BatchItem), 256);
+          o += align_up(layers[l].direct.items.size() * sizeof(BatchItem), 256);
+          for (int k = 1; k <= kMaxWidth; ++k) {
+               o_tile[l][k] = o;
+               o += align_up(layers[l].tile[k].items.size() * sizeof(BatchTileItem), 256);
+          }
      }
      const size_t o_sums = o;
      o += align_up(sums.items.size() * sizeof(BatchItem), 256);
@@ -440,9 +481,13 @@ std::vector<int64_t> Server::step(bool block)
      const size_t desc_bytes = std::max<size_t>(o, 256);
      ensure_slot(slot, desc_bytes, std::max<size_t>(out_bytes, 256));
      std::memcpy(slot.h_desc + o_init, init.items.data(), init.items.size() * sizeof(BatchItem));
-     for (int l = 0; l < pps_; ++l)
-          std::memcpy(slot.h_desc + o_layer[l], layers[l].items.data(),
-                      layers[l].items.size() * sizeof(BatchItem));
+     for (int l = 0; l < pps_; ++l) {
+          std::memcpy(slot.h_desc + o_layer[l], layers[l].direct.items.data(),
+                      layers[l].direct.items.size() * sizeof(BatchItem));
+          for (int k = 1; k <= kMaxWidth; ++k)
+               std::memcpy(slot.h_desc + o_tile[l][k], layers[l].tile[k].items.data(),
+                           layers[l].tile[k].items.size() * sizeof(BatchTileItem));
+     }
      std::memcpy(slot.h_desc + o_sums, sums.items.data(), sums.items.size() * sizeof(BatchItem));
      std::memcpy(slot.h_desc + o_jobs, jobs.data(), jobs.size() * sizeof(SampleJob));
 
@@ -464,10 +509,19 @@ std::vector<int64_t> Server::step(bool block)
           ++stats_.launches;
      }
      for (int l = 0; l < pps_; ++l) {
-          if (layers[l].items.empty()) continue;
+          for (int k = 1; k <= kMaxWidth; ++k) {
+               const TileList& tl = layers[l].tile[k];
+               if (tl.items.empty()) continue;
+               check(launch_batch_tile(d_states_, prec_, d_mats_,
+                                       reinterpret_cast<const BatchTileItem*>(slot.d_desc + o_tile[l][k]),
+                                       static_cast<int>(tl.items.size()), k, tl.blocks, stream_),
+                     "launch_batch_tile");
+               ++stats_.launches;
+          }
+          const ItemList& dl = layers[l].direct;
+          if (dl.items.empty()) continue;
           check(launch_serve_layer(d_states_, prec_, d_mats_, d_items(o_layer[l]),
-                                   static_cast<int>(layers[l].items.size()), layers[l].blocks, kmax,
-                                   stream_),
+                                   static_cast<int>(dl.items.size()), dl.blocks, kmax, stream_),
                 "launch_serve_layer");
           ++stats_.launches;
      }

@@ -128,6 +128,7 @@ private:
      size_t sums_cap_ = 0;
      RangeAlloc state_pool_, mat_pool_;
      uint64_t state_cap_amps_ = 0;
+     uint64_t mat_cap_entries_ = 0;
      std::vector<int> k_by_n_;
      std::map<int64_t, Req> reqs_;
      std::deque<int64_t> waiting_;

@@ -210,7 +210,11 @@ int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int
  * layered many-workgroup launches, the others each by one workgroup of a
  * persistent launch. */
 int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits);
-/* Plans, fuses, uploads and captures the launch graph. */
+/* Plans and fuses every circuit and checks every launch's blocks and
+ * addresses against its buffers on the host; needs no device (stats are
+ * filled as for a build on a 256-CU device). */
+int hiqsim_batch_prepare(hiqsim_batch_t* b);
+/* Plans, fuses, checks, uploads and captures the launch graph. */
 int hiqsim_batch_build(hiqsim_batch_t* b);
 /* Runs every circuit from |0...0> (graph replay); *device_ms = device time. */
 int hiqsim_batch_run(hiqsim_batch_t* b, double* device_ms);
@@ -218,9 +222,10 @@ int hiqsim_batch_num_circuits(hiqsim_batch_t* b, int64_t* n);
 int hiqsim_batch_num_qubits(hiqsim_batch_t* b, int64_t index, int* n_qubits);
 /* Final state of circuit `index`: 2^n interleaved complex128 values. */
 int hiqsim_batch_get_state(hiqsim_batch_t* b, int64_t index, double* out);
-/* out[0..9] = circuits, fused passes, workgroup-owned circuits, LDS-resident
+/* out[0..10] = circuits, fused passes, workgroup-owned circuits, LDS-resident
  * circuits, layered launches, arena bytes, matrix bytes, amplitude-passes
- * (sum over passes of 2^n), persistent workgroups, widest fusion width. */
+ * (sum over passes of 2^n), persistent workgroups, widest fusion width,
+ * layered launches whose 2-D grid carries padding blocks. */
 int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len);
 
 /* ---- request loop (continuous batching) ---------------------------------
