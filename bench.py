@@ -177,8 +177,9 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
         tune = load_tuning(prec)
         k_by_n = tune["k_by_n"] if tune else {}
         wg_max = tune["wg_max_qubits"] if tune else 16
+        tile_min = tune["tile_min_qubits"] if tune else 0
         t0 = time.perf_counter()
-        b = make_batch(circuits, prec, k_by_n, wg_max)
+        b = make_batch(circuits, prec, k_by_n, wg_max, tile_min_qubits=tile_min)
         build_s = time.perf_counter() - t0
         b.run()
         ms = min(b.run() for _ in range(3))
@@ -191,7 +192,7 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
                      "lds_circuits": st["lds_circuits"], "layer_launches": st["layer_launches"],
                      "arena_gib": round(st["arena_bytes"] / 2 ** 30, 2), "build_s": round(build_s, 2),
                      "plan": "autotuned table" if tune else "default (k=4, wg_max_qubits=16)",
-                     "wg_max_qubits": wg_max}
+                     "wg_max_qubits": wg_max, "tile_min_qubits": tile_min}
         log_fn("batch %s: %s" % (prec, out[prec]))
         del b
     fids = [fidelity(a, b) for a, b in zip(states["c128"], states["c64"])]
@@ -258,6 +259,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rates", default="20,60")
+    ap.add_argument("--block-swizzle", type=int, default=None,
+                    help="low block-index bits reversed in the streaming passes (library default 8)")
+    ap.add_argument("--tile-high-target", type=int, default=None,
+                    help="dense passes with every target at or above this bit use the direct form")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()
@@ -267,6 +272,11 @@ def main():
         sys.exit(2)
 
     import hiqsimulator_amd as hq
+    if args.block_swizzle is not None:
+        hq._cppsim_amd.set_tuning("block_swizzle", args.block_swizzle)
+    if args.tile_high_target is not None:
+        hq._cppsim_amd.set_tuning("tile_high_target", args.tile_high_target)
+    tuning = {name: hq._cppsim_amd.get_tuning(name) for name in ("block_swizzle", "tile_high_target")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
     gates, step = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
@@ -340,7 +350,7 @@ def main():
                    "planner": args.planner,
                    "qubits": n, "layers": args.layers, "fusion_k": k, "passes_per_step": n_pass,
                    "permutations_per_step": n_perm, "plan_seconds": round(plan_s, 3),
-                   "final_norm": norm},
+                   "final_norm": norm, "kernel_tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom) if n == 30 else None,

@@ -9,7 +9,10 @@ knobs, both picked here by measurement:
 * ``wg_max_qubits``: circuits of at most this many qubits are each owned by
   one workgroup of a persistent launch (LDS-resident up to 128 KiB of
   state), larger ones are advanced by layered launches with many workgroups
-  per state.
+  per state;
+* ``tile_min_qubits``: layered passes of circuits from this many qubits on
+  may take the wave-tile form (LDS regrouping, non-temporal streaming);
+  smaller ones use the direct form.
 """
 import json
 import os
@@ -36,6 +39,7 @@ def load_tuning(precision, arch="gfx950"):
     with open(path) as f:
         t = json.load(f)
     t["k_by_n"] = {int(n): int(k) for n, k in t["k_by_n"].items()}
+    t.setdefault("tile_min_qubits", 0)
     return t
 
 
@@ -44,6 +48,7 @@ def save_tuning(result, precision, arch="gfx950"):
     with open(tuning_path(precision, arch), "w") as f:
         json.dump({"k_by_n": {str(n): k for n, k in sorted(result["k_by_n"].items())},
                    "wg_max_qubits": result["wg_max_qubits"],
+                   "tile_min_qubits": result.get("tile_min_qubits", 0),
                    "probe_ms": result.get("probe_ms", {})}, f, indent=1, sort_keys=True)
 
 
@@ -60,12 +65,12 @@ def batch_workload(count, seed, n_range=(10, 22), depth_range=(10, 60), haar_eve
 
 
 def make_batch(circuits, precision="c128", k_by_n=None, wg_max_qubits=DEFAULT_WG_MAX_QUBITS,
-               device=0, build=True):
+               device=0, build=True, tile_min_qubits=0):
     """A ``Batch`` holding ``circuits`` ([(n, gates), ...]), planned and built."""
     b = Batch(precision, device)
     for n, gates in circuits:
         b.add_circuit(n, gates)
-    b.set_plan(dict(k_by_n or {}), int(wg_max_qubits))
+    b.set_plan(dict(k_by_n or {}), int(wg_max_qubits), int(tile_min_qubits))
     if build:
         b.build()
     return b
@@ -83,10 +88,13 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
     For every n in ``ns`` a probe batch of copies of one depth-``depth``
     random circuit (enough copies to hold about ``amp_budget`` amplitudes,
     8..2048) is timed for every width in ``widths``, on the persistent workgroup
-    path (n <= ``wg_limit``) and on the layered path (n >= ``layer_floor``).
-    Returns ``{"k_by_n": {n: k}, "wg_max_qubits": t, "probe_ms": {...}}``
-    where t is the largest n whose best workgroup time beats its best
-    layered time (the split is monotone in n by construction).
+    path (n <= ``wg_limit``) and on the layered path (n >= ``layer_floor``)
+    in its direct and wave-tile forms.
+    Returns ``{"k_by_n": {n: k}, "wg_max_qubits": t, "tile_min_qubits": m,
+    "probe_ms": {...}}`` where t is the largest n whose best workgroup time
+    beats its best layered time (the split is monotone in n by
+    construction) and m the smallest n from which the tile form wins for
+    every larger layered n.
     """
     ns = sorted(set(int(n) for n in ns))
     best = {}
@@ -96,15 +104,15 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
         b = Batch(precision, device)
         b.add_circuit(n, random_layers_circuit(n, depth, seed + n, haar_every=5))
         b.add_copies(0, copies - 1)
-        for path, limit in (("wg", 24), ("layer", 0)):
+        for path, limit, tmin in (("wg", 24, 31), ("layer", 0, 31), ("tile", 0, 0)):
             if path == "wg" and n > wg_limit:
                 continue
-            if path == "layer" and n < layer_floor:
+            if path != "wg" and n < layer_floor:
                 continue
             for k in widths:
                 if k > n:
                     continue
-                b.set_plan({n: k}, limit)
+                b.set_plan({n: k}, limit, tmin)
                 b.build()
                 ms = _time_batch(b, reps)
                 probe[(n, path, k)] = ms
@@ -113,19 +121,33 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
                 if (n, path) not in best or ms < best[(n, path)][1]:
                     best[(n, path)] = (k, ms)
         del b
+    def layered(n):
+        cands = [best[(n, p)] + (p,) for p in ("layer", "tile") if (n, p) in best]
+        return min(cands, key=lambda c: c[1]) if cands else None
+
     wg_max = 0
     for n in ns:
         wg = best.get((n, "wg"))
-        ly = best.get((n, "layer"))
+        ly = layered(n)
         if wg and (ly is None or wg[1] <= ly[1]):
             wg_max = n
         else:
             break
+    tile_min = 31
+    for n in reversed([n for n in ns if n > wg_max]):
+        ly = layered(n)
+        if ly is not None and ly[2] == "tile":
+            tile_min = n
+        else:
+            break
     k_by_n = {}
     for n in ns:
-        path = "wg" if n <= wg_max else "layer"
-        k_by_n[n] = best[(n, path)][0] if (n, path) in best else 4
-    return {"k_by_n": k_by_n, "wg_max_qubits": wg_max,
+        if n <= wg_max:
+            k_by_n[n] = best[(n, "wg")][0] if (n, "wg") in best else 4
+        else:
+            form = "tile" if n >= tile_min else "layer"
+            k_by_n[n] = best[(n, form)][0] if (n, form) in best else 4
+    return {"k_by_n": k_by_n, "wg_max_qubits": wg_max, "tile_min_qubits": tile_min,
             "probe_ms": {"%d/%s/k%d" % key: round(v, 4) for key, v in probe.items()}}
 
 

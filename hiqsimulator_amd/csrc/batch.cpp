@@ -196,14 +196,17 @@ int64_t Batch::add_copies(int64_t c, int64_t count)
      return first;
 }
 
-void Batch::set_plan(const std::vector<int>& k_by_n, int wg_max_qubits)
+void Batch::set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits)
 {
+     if (tile_min_qubits < 0 || tile_min_qubits > 31)
+          throw std::runtime_error("batch: tile_min_qubits must be 0..31");
      for (int k: k_by_n)
           if (k < 0 || k > kMaxWidth) throw std::runtime_error("batch: fusion width must be 0..5");
      if (wg_max_qubits < 0 || wg_max_qubits > kMaxWgQubits)
           throw std::runtime_error("batch: wg_max_qubits must be 0.." + std::to_string(kMaxWgQubits));
      k_by_n_ = k_by_n;
      wg_max_qubits_ = wg_max_qubits;
+     tile_min_qubits_ = tile_min_qubits;
      built_ = false;
 }
 
@@ -352,7 +355,7 @@ Batch::HostPlan Batch::plan_host(int workgroups)
                     const PlannedPass& pp = plan[c][L];
                     const int n = circ_[c].n;
                     BatchTileItem ti{};
-                    const bool tile_pays = pp.pos[0] < 20 &&
+                    const bool tile_pays = n >= tile_min_qubits_ && pp.pos[0] < get_tile_high_target() &&
                                            (k < 5 || pp.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
                     if (tile_pays && make_tile_item(n, k, pp.pos, prec_, ti)) {
                          ti.state_off = circ_[c].state_off;
@@ -449,7 +452,8 @@ void Batch::check_plan(const HostPlan& hp) const
                     const int n = hp.titem_n[i];
                     const int wpb = batch_tile_wpb(ll.k, prec_);
                     const uint64_t nb = (uint64_t(1) << (n - ll.k)) / (64ull * wpb);
-                    if (it.block_start != next || nb == 0 || nb * 64ull * wpb != (uint64_t(1) << (n - ll.k)))
+                    if (it.block_start != next || nb == 0 || nb * 64ull * wpb != (uint64_t(1) << (n - ll.k)) ||
+                        (uint64_t(1) << it.log_blocks) != nb)
                          plan_error("tile item blocks");
                     next += nb;
                     if (it.nins != ll.k || !pos_ok(it.ins, ll.k, n) || !state_ok(it.state_off, n) ||

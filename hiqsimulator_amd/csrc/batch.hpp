@@ -72,7 +72,8 @@ struct BatchTileItem {
      uint8_t nins;
      uint8_t rank_n[6];
      uint8_t rank_t[5];
-     uint8_t pad[7];
+     uint8_t log_blocks;  // log2 of the item's blocks
+     uint8_t pad[6];
 };
 
 // Waves per block of the batched tile kernel (LDS holds 64 * 2^k amplitudes
@@ -179,8 +180,9 @@ public:
      // Appends `count` copies of circuit c; returns the first copy's index.
      int64_t add_copies(int64_t c, int64_t count);
      // k_by_n[n] = fusion width for circuits of n qubits (0 = default 4);
-     // circuits of more than wg_max_qubits qubits take the layered path.
-     void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits);
+     // circuits of more than wg_max_qubits qubits take the layered path, whose
+     // passes may use the wave-tile form from tile_min_qubits qubits on.
+     void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits);
      // Plans and fuses every circuit and lays out the device descriptors on
      // the host, then checks every launch's blocks and addresses against the
      // arena, matrix and item bounds (throws on a violation).  Needs no device.
@@ -239,6 +241,7 @@ private:
      std::vector<Circ> circ_;
      std::vector<int> k_by_n_;
      int wg_max_qubits_ = 16;
+     int tile_min_qubits_ = 0;
      bool built_ = false;
      bool device_ready_ = false;
      void* d_arena_ = nullptr;

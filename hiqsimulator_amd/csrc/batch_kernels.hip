@@ -143,7 +143,8 @@ __global__ void __launch_bounds__(NT)
 template <int K, typename R>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_batch_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
-                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
+                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks,
+                  const int swz)
 {
      using V = VT<R>;
      constexpr int D = 1 << K;
@@ -156,7 +157,8 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
           else hi = mid - 1;
      }
      const BatchItem it = items[lo];
-     const uint64_t t = (b - it.block_start) * kBatchLayerBlock + threadIdx.x;
+     const int lbk = it.log_groups > 8 ? static_cast<int>(it.log_groups) - 8 : 0;
+     const uint64_t t = swizzle_block(b - it.block_start, swz < lbk ? swz : lbk) * kBatchLayerBlock + threadIdx.x;
      if ((t >> it.log_groups) != 0) return;
      uint64_t base = t;
      uint64_t off[D];
@@ -225,12 +227,14 @@ __device__ __forceinline__ void layer_group(VT<R>* __restrict__ arena, const VT<
 template <typename R, int KMAX>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_serve_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
-                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
+                  const BatchItem* __restrict__ items, const int nitems, const uint64_t total_blocks,
+                  const int swz)
 {
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid
      const BatchItem it = items[find_item(items, nitems, b)];
-     const uint64_t t = (b - it.block_start) * kBatchLayerBlock + threadIdx.x;
+     const int lbk = it.log_groups > 8 ? static_cast<int>(it.log_groups) - 8 : 0;
+     const uint64_t t = swizzle_block(b - it.block_start, swz < lbk ? swz : lbk) * kBatchLayerBlock + threadIdx.x;
      if ((t >> it.log_groups) != 0) return;
      switch (it.k) {
           case 1: layer_group<1, R>(arena, mats, it, t); break;
@@ -248,7 +252,8 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
 template <int K, typename R, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
     k_batch_tile(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
-                 const BatchTileItem* __restrict__ items, const int nitems, const uint64_t total_blocks)
+                 const BatchTileItem* __restrict__ items, const int nitems, const uint64_t total_blocks,
+                 const int swz)
 {
      using V = VT<R>;
      constexpr int D = 1 << K;
@@ -258,7 +263,8 @@ __global__ void __launch_bounds__(64 * WPB)
      const BatchTileItem& it = items[find_item(items, nitems, b)];
      const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
      V* const T = tile[wv];
-     uint64_t idx = ((b - it.block_start) * WPB + wv) << 6;
+     const uint64_t lb = swizzle_block(b - it.block_start, swz < it.log_blocks ? swz : it.log_blocks);
+     uint64_t idx = (lb * WPB + wv) << 6;
 #pragma unroll
      for (int l = 0; l < K; ++l) {
           const uint64_t lo_bits = idx & ((1ull << it.ins[l]) - 1ull);
@@ -451,11 +457,12 @@ hipError_t launch_batch_layer(void* arena, Prec prec, const void* mats, const Ba
           if (prec == Prec::C128)                                                                 \
                hipLaunchKernelGGL((k_batch_layer<KK, double>), grid, dim3(kBatchLayerBlock), 0,   \
                                   s, static_cast<double2*>(arena),                                \
-                                  static_cast<const double2*>(mats), items, nitems, total_blocks); \
+                                  static_cast<const double2*>(mats), items, nitems, total_blocks, \
+                                  get_block_swizzle());                                           \
           else                                                                                    \
                hipLaunchKernelGGL((k_batch_layer<KK, float>), grid, dim3(kBatchLayerBlock), 0, s, \
                                   static_cast<float2*>(arena), static_cast<const float2*>(mats),  \
-                                  items, nitems, total_blocks);                                   \
+                                  items, nitems, total_blocks, get_block_swizzle());              \
      } while (0)
      switch (k) {
           case 1: HQ_LAYER(1); break;
@@ -478,7 +485,7 @@ hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const Ba
 #define HQ_SERVE(RR, KM)                                                                         \
      hipLaunchKernelGGL((k_serve_layer<RR, KM>), grid, dim3(kBatchLayerBlock), 0, s,              \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
-                        nitems, total_blocks)
+                        nitems, total_blocks, get_block_swizzle())
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_SERVE(double, 5);
           else HQ_SERVE(double, 4);
@@ -522,6 +529,7 @@ bool make_tile_item(int n, int k, const uint8_t* pos, Prec prec, BatchTileItem& 
           it.ins[l] = pos[l];
      }
      it.nins = static_cast<uint8_t>(k);
+     it.log_blocks = static_cast<uint8_t>(n - k - 6 - (wpb == 4 ? 2 : 1));
      for (int i = 0; i < 32; ++i) {
           uint64_t o = 0;
           if (i < (1 << k))
@@ -543,13 +551,13 @@ hipError_t launch_batch_tile(void* arena, Prec prec, const void* mats, const Bat
                constexpr int W = (64 << KK) * 16 > 16384 ? 2 : 4;                                 \
                hipLaunchKernelGGL((k_batch_tile<KK, double, W>), grid, dim3(64 * W), 0, s,        \
                                   static_cast<double2*>(arena), static_cast<const double2*>(mats), \
-                                  items, nitems, total_blocks);                                   \
+                                  items, nitems, total_blocks, get_block_swizzle());              \
           }                                                                                       \
           else {                                                                                  \
                constexpr int W = (64 << KK) * 8 > 16384 ? 2 : 4;                                  \
                hipLaunchKernelGGL((k_batch_tile<KK, float, W>), grid, dim3(64 * W), 0, s,         \
                                   static_cast<float2*>(arena), static_cast<const float2*>(mats),   \
-                                  items, nitems, total_blocks);                                   \
+                                  items, nitems, total_blocks, get_block_swizzle());              \
           }                                                                                       \
      } while (0)
      switch (k) {

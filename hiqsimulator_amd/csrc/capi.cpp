@@ -484,6 +484,36 @@ int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const 
      });
 }
 
+int hiqsim_set_tuning(const char* name, int64_t value)
+{
+     return guard([&] {
+          need(name, "name");
+          const std::string n(name);
+          if (n == "tile_high_target") {
+               if (value < 0 || value > 64) throw std::runtime_error("tile_high_target must be 0..64");
+               hq::set_tile_high_target(static_cast<int>(value));
+          }
+          else if (n == "block_swizzle") {
+               if (value < 0 || value > 16) throw std::runtime_error("block_swizzle must be 0..16");
+               hq::set_block_swizzle(static_cast<int>(value));
+          }
+          else
+               throw std::runtime_error("unknown tuning parameter '" + n + "'");
+     });
+}
+
+int hiqsim_get_tuning(const char* name, int64_t* value)
+{
+     return guard([&] {
+          need(name, "name");
+          need(value, "value");
+          const std::string n(name);
+          if (n == "tile_high_target") *value = hq::get_tile_high_target();
+          else if (n == "block_swizzle") *value = hq::get_block_swizzle();
+          else throw std::runtime_error("unknown tuning parameter '" + n + "'");
+     });
+}
+
 int hiqsim_batch_create(int precision, int device, hiqsim_batch_t** out)
 {
      return guard([&] {
@@ -520,12 +550,13 @@ int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int
      });
 }
 
-int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits)
+int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits,
+                          int tile_min_qubits)
 {
      return guard([&] {
           need(b, "batch");
           if (len) need(k_by_n, "k_by_n");
-          b->b.set_plan(std::vector<int>(k_by_n, k_by_n + len), wg_max_qubits);
+          b->b.set_plan(std::vector<int>(k_by_n, k_by_n + len), wg_max_qubits, tile_min_qubits);
      });
 }
 

@@ -150,6 +150,23 @@ __device__ __forceinline__ int tile_slot(int r)
      return r ^ ((r >> S) & ((1 << S) - 1));
 }
 
+// Block order of the streaming passes.  Blocks start in index order, so with
+// the identity mapping the blocks in flight at one time cover one narrow
+// address range inside each of the 2^k chunk strides of a pass; measured on
+// MI355X this concentrates the traffic on few HBM channels when the strides
+// are large (targets at bits 18-29: 4.0-4.6 TB/s for a 30-qubit k = 4 pass
+// against 6.0-6.6 TB/s at low bits).  Reversing the low `bits` bits of the
+// block index spreads consecutive blocks over 2^bits far-apart address
+// ranges; it is a bijection of [0, nblocks) when nblocks is a multiple of
+// 2^bits.
+__device__ __forceinline__ uint64_t swizzle_block(uint64_t b, int bits)
+{
+     if (bits <= 0) return b;
+     const uint64_t mask = (1ull << bits) - 1ull;
+     const uint32_t r = __builtin_bitreverse32(static_cast<uint32_t>(b & mask)) >> (32 - bits);
+     return (b & ~mask) | r;
+}
+
 // 53-bit uniform in [0, 1) from two 32-bit Philox outputs.
 __device__ __forceinline__ double uniform53(uint32_t hi, uint32_t lo)
 {

@@ -35,7 +35,7 @@ namespace {
 template <int K, typename R, int GPT, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_dense_direct(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
-                   const DenseDesc d, const uint64_t ngroups)
+                   const DenseDesc d, const uint64_t ngroups, const int swz)
 {
      using V = typename C2<R>::T;
      constexpr int D = 1 << K;
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(kBlock)
                if ((j >> l) & 1) o += 1ull << d.pos[l];
           off[j] = o;
      }
-     const uint64_t t0 = block_linear() * (kBlock * GPT) + threadIdx.x;
+     const uint64_t t0 = swizzle_block(block_linear(), swz) * (kBlock * GPT) + threadIdx.x;
      uint64_t I[GPT];
      V v[GPT][D];
 #pragma unroll
@@ -82,14 +82,14 @@ struct TileDesc {
 template <int K, typename R, int WPB>
 __global__ void __launch_bounds__(64 * WPB)
     k_dense_tile(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
-                 const DenseDesc d, const TileDesc L)
+                 const DenseDesc d, const TileDesc L, const int swz)
 {
      using V = typename C2<R>::T;
      constexpr int D = 1 << K;
      __shared__ V tile[WPB][64 * D];
      const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
      V* const T = tile[wv];
-     const uint64_t wave = block_linear() * WPB + wv;
+     const uint64_t wave = swizzle_block(block_linear(), swz) * WPB + wv;
      V* const base = psi + (insert_zeros(wave << 6, d.ins) | d.ctrlmask) + lane;
      V v[D];
 #pragma unroll
@@ -589,6 +589,17 @@ constexpr int tile_wpb()
 // Target bits at or above this position make the direct form faster than
 // the tile form (their chunks lie >= 2^g_high_target amplitudes apart).
 int g_high_target = 20;
+// Low block-index bits reversed by swizzle_block in the dense passes.
+int g_swizzle_bits = 8;
+
+// Swizzle width for a grid of `blocks` blocks: the configured width when the
+// grid holds whole runs of 2^bits blocks, else none.
+int swizzle_bits_for(uint64_t blocks)
+{
+     const int bits = g_swizzle_bits;
+     if (bits <= 0 || bits > 16) return 0;
+     return (blocks % (1ull << bits)) == 0 ? bits : 0;
+}
 
 template <int K, typename R>
 hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStream_t s)
@@ -642,7 +653,8 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
           }
           const uint64_t blocks = ngroups / (64ull * WPB);
           hipLaunchKernelGGL((k_dense_tile<K, R, WPB>), grid_2d(blocks), dim3(64 * WPB), 0, s,
-                             static_cast<V*>(psi), static_cast<const V*>(mat), d, L);
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, L,
+                             swizzle_bits_for(blocks));
           return hipGetLastError();
      }
      // Direct form; non-temporal when every lane access covers whole 128-byte
@@ -650,13 +662,16 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      constexpr int GPT = K >= 4 ? 1 : (K == 3 ? 4 : (K == 2 ? 8 : 16));
      const int line_bits = sizeof(V) == 16 ? 3 : 4;
      const bool nt = (ins_mask & ((1ull << line_bits) - 1)) == 0;
-     const dim3 grid = grid_2d((ngroups + kBlock * GPT - 1) / (kBlock * GPT));
+     const uint64_t dblocks = (ngroups + kBlock * GPT - 1) / (kBlock * GPT);
+     const dim3 grid = grid_2d(dblocks);
+     // whole blocks only (a partial last block must stay last)
+     const int swz = (ngroups % (kBlock * GPT)) == 0 ? swizzle_bits_for(dblocks) : 0;
      if (nt)
           hipLaunchKernelGGL((k_dense_direct<K, R, GPT, true>), grid, dim3(kBlock), 0, s,
-                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups);
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups, swz);
      else
           hipLaunchKernelGGL((k_dense_direct<K, R, GPT, false>), grid, dim3(kBlock), 0, s,
-                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups);
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups, swz);
      return hipGetLastError();
 }
 
@@ -733,6 +748,9 @@ hipError_t reduce_launch(uint64_t work, double* scratch, double* out, hipStream_
 }  // namespace
 
 void set_tile_high_target(int pos) { g_high_target = pos; }
+int get_tile_high_target() { return g_high_target; }
+void set_block_swizzle(int bits) { g_swizzle_bits = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
+int get_block_swizzle() { return g_swizzle_bits; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -41,6 +41,11 @@ hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat,
 // The dense pass takes its direct form (instead of the LDS tile form) when
 // every target bit is at or above this position (default 20).
 void set_tile_high_target(int pos);
+int get_tile_high_target();
+// Low block-index bits reversed in the dense passes (0 = identity order,
+// default 8, at most 16).
+void set_block_swizzle(int bits);
+int get_block_swizzle();
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

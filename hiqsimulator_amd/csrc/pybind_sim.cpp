@@ -487,7 +487,7 @@ public:
           ok(hiqsim_batch_add_copies(h_, index, count, &first));
           return first;
      }
-     void set_plan(const std::map<int, int>& k_by_n, int wg_max_qubits)
+     void set_plan(const std::map<int, int>& k_by_n, int wg_max_qubits, int tile_min_qubits)
      {
           std::vector<int32_t> v;
           for (auto& kv: k_by_n) {
@@ -495,7 +495,7 @@ public:
                if (v.size() <= static_cast<size_t>(kv.first)) v.resize(kv.first + 1, 0);
                v[kv.first] = kv.second;
           }
-          ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits));
+          ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits, tile_min_qubits));
      }
      void prepare()
      {
@@ -711,6 +711,12 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("set_timing", &PySim::set_timing)
          .def("kernel_time", &PySim::kernel_time)
          .def("state_size", &PySim::state_size);
+     m.def("set_tuning", [](const std::string& name, int64_t v) { ok(hiqsim_set_tuning(name.c_str(), v)); });
+     m.def("get_tuning", [](const std::string& name) {
+          int64_t v = 0;
+          ok(hiqsim_get_tuning(name.c_str(), &v));
+          return v;
+     });
      m.def("plan_clusters",
            [](const std::vector<std::vector<int64_t>>& gate,
               const std::vector<std::vector<int64_t>>& ctrl, int cluster_size, int mode) {
@@ -748,7 +754,8 @@ PYBIND11_MODULE(_cppsim_amd, m)
               py::arg("device") = 0)
          .def("add_circuit", &PyBatch::add_circuit, py::arg("n_qubits"), py::arg("gates"))
          .def("add_copies", &PyBatch::add_copies, py::arg("index"), py::arg("count"))
-         .def("set_plan", &PyBatch::set_plan, py::arg("k_by_n"), py::arg("wg_max_qubits") = 16)
+         .def("set_plan", &PyBatch::set_plan, py::arg("k_by_n"), py::arg("wg_max_qubits") = 16,
+              py::arg("tile_min_qubits") = 0)
          .def("prepare", &PyBatch::prepare)
          .def("build", &PyBatch::build)
          .def("run", &PyBatch::run)

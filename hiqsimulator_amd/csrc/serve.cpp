@@ -393,7 +393,8 @@ std::vector<int64_t> Server::step(bool block)
           for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
                const Pass& p = r.passes[r.next + l];
                const uint64_t moff = r.mat_off + r.pass_mat[r.next + l];
-               const bool tile_pays = p.pos[0] < 20 && (p.k < 5 || p.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
+               const bool tile_pays = p.pos[0] < get_tile_high_target() &&
+                                      (p.k < 5 || p.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
                if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ ||
                    moff + (uint64_t(1) << (2 * p.k)) > mat_cap_entries_ || p.k < 1 || p.k > r.n)
                     throw std::runtime_error("Server: internal bounds check failed");
@@ -407,7 +408,8 @@ std::vector<int64_t> Server::step(bool block)
                          const uint64_t lo = last & ((uint64_t(1) << p.pos[q]) - 1);
                          last = ((last ^ lo) << 1) | lo;
                     }
-                    if (nb == 0 || last + 63 + ti.hoff[(1 << p.k) - 1] >= (uint64_t(1) << r.n))
+                    if (nb == 0 || (uint64_t(1) << ti.log_blocks) != nb ||
+                        last + 63 + ti.hoff[(1 << p.k) - 1] >= (uint64_t(1) << r.n))
                          throw std::runtime_error("Server: internal tile bounds check failed");
                     layers[l].tile[p.k].add(ti, nb);
                     continue;

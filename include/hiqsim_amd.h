@@ -183,6 +183,15 @@ int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const 
                          const size_t* ctrl_off, size_t ngates, int cluster_size, int mode,
                          int32_t* cluster_of);
 
+/* Process-wide kernel tuning (no reference counterpart):
+ *   "tile_high_target": dense passes whose targets all sit at or above this
+ *                       bit take the direct form (default 20);
+ *   "block_swizzle":    low block-index bits reversed in the streaming
+ *                       passes (0 = identity order, default 8, at most 16).
+ * Batches and request loops read them when they build or launch. */
+int hiqsim_set_tuning(const char* name, int64_t value);
+int hiqsim_get_tuning(const char* name, int64_t* value);
+
 /* ---- batches of independent circuits ------------------------------------
  * No counterpart in the reference, which simulates one circuit per
  * SimulatorMPI object.  Each circuit of a batch starts from |0...0> on its
@@ -208,8 +217,10 @@ int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int
 /* k_by_n[n] = fusion width (1..5, 0 = default 4) for circuits of n qubits;
  * circuits of more than wg_max_qubits (0..24) qubits are advanced by
  * layered many-workgroup launches, the others each by one workgroup of a
- * persistent launch. */
-int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits);
+ * persistent launch.  Layered passes of circuits of at least
+ * tile_min_qubits qubits may take the wave-tile form (31 = never). */
+int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits,
+                          int tile_min_qubits);
 /* Plans and fuses every circuit and checks every launch's blocks and
  * addresses against its buffers on the host; needs no device (stats are
  * filled as for a build on a 256-CU device). */

@@ -38,20 +38,21 @@ def oracle_state(oracle_mod, n, gates):
 
 
 PLANS = {
-    # name: (k_by_n, wg_max_qubits)
-    "k4_split14": ({}, 14),
-    "kvar_all_wg": ({n: 2 + n % 4 for n in range(1, 18)}, 24),
-    "k5_all_layered": ({n: 5 for n in range(1, 18)}, 0),
-    "k2_split16": ({n: 2 for n in range(1, 18)}, 16),
+    # name: (k_by_n, wg_max_qubits, tile_min_qubits)
+    "k4_split14": ({}, 14, 0),
+    "kvar_all_wg": ({n: 2 + n % 4 for n in range(1, 18)}, 24, 0),
+    "k5_all_layered": ({n: 5 for n in range(1, 18)}, 0, 0),
+    "k2_split16": ({n: 2 for n in range(1, 18)}, 16, 0),
+    "k3_layered_direct": ({n: 3 for n in range(1, 18)}, 10, 31),
 }
 
 
 @pytest.mark.parametrize("plan", sorted(PLANS))
 def test_batch_c128_matches_oracle(engine_mod, oracle_mod, plan):
-    k_by_n, wg_max = PLANS[plan]
+    k_by_n, wg_max, tile_min = PLANS[plan]
     circuits = mixed_circuits(5)
     print("batch c128 plan", plan, flush=True)
-    b = make_batch(circuits, "c128", k_by_n, wg_max)
+    b = make_batch(circuits, "c128", k_by_n, wg_max, tile_min_qubits=tile_min)
     st = b.stats()
     assert st["circuits"] == len(circuits)
     b.run()
@@ -117,5 +118,7 @@ def test_batch_errors(engine_mod):
         b.add_circuit(3, [(X, [0], [0])])  # repeated qubit
     with pytest.raises(RuntimeError):
         b.set_plan({4: 6}, 16)
+    with pytest.raises(RuntimeError):
+        b.set_plan({4: 4}, 16, 32)
     with pytest.raises(RuntimeError):
         hiqsimulator_amd.Batch("c32")

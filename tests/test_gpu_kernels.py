@@ -137,3 +137,31 @@ def test_dense_forms_22q(engine_mod, oracle_mod, precision):
         oracle_mod.dense_pass(ref, m, pos, cm)
         err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
         assert err <= tol, (pos, cm, err)
+
+
+@pytest.mark.parametrize("swizzle,high", [(0, 20), (4, 20), (8, 64), (10, 64), (16, 0)])
+def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
+    """Block-order swizzle and the tile/direct threshold change only the
+    order and form of the work: results equal the oracle at every setting."""
+    import hiqsimulator_amd
+    mod = hiqsimulator_amd._cppsim_amd
+    old = (mod.get_tuning("block_swizzle"), mod.get_tuning("tile_high_target"))
+    n = 22
+    rng = np.random.default_rng(7 + swizzle)
+    st = rand_state(rng, n)
+    try:
+        mod.set_tuning("block_swizzle", swizzle)
+        mod.set_tuning("tile_high_target", high)
+        print("swizzle", swizzle, "high", high, flush=True)
+        for pos in ([0, 1, 2, 3], [18, 19, 20, 21], [2, 9, 15, 21], [20], [0, 5, 10, 15, 20]):
+            k = len(pos)
+            m = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+            m /= np.linalg.norm(m, 2)
+            sim = engine_with_state(engine_mod, st, "c128")
+            sim.apply_matrix(m, pos, 0, False)
+            ref = st.copy()
+            oracle_mod.dense_pass(ref, m, pos, 0)
+            assert np.abs(sim.get_state() - ref).max() <= 1e-12 * np.abs(ref).max(), pos
+    finally:
+        mod.set_tuning("block_swizzle", old[0])
+        mod.set_tuning("tile_high_target", old[1])
