@@ -90,17 +90,33 @@ def build_step(n, layers, k, seed, n_perm, planner="greedy"):
             stretch = []
             step.append(("perm", payload))
     step.extend(("pass", p) for p in plan(stretch))
-    return gates, step
+    # the same circuit for tile passes: each stretch between permutations is
+    # one apply_tiled call
+    tiled, stretch = [], []
+    for kind, payload in ops:
+        if kind == "gates":
+            stretch.extend(payload)
+        else:
+            tiled.append(("tiled", stretch))
+            stretch = []
+            tiled.append(("perm", payload))
+    tiled.append(("tiled", stretch))
+    return gates, step, tiled
 
 
-def run_step(sim, step):
+def run_step(sim, step, op_width=4):
+    passes = 0
     for kind, payload in step:
         if kind == "pass":
             for m, t, c in payload:
                 sim.apply_controlled_gate(m, t, c)
             sim.run()
+            passes += 1
+        elif kind == "tiled":
+            passes += sim.apply_tiled(payload, op_width)
         else:
             sim.permute_qubits(payload)
+    return passes
 
 
 def kernel_sweep(hq, n, reps=6):
@@ -254,6 +270,7 @@ def main():
                     help="greedy: the GreedyScheduler/ClusterScheduler loop; inorder: program order")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-tile", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
@@ -279,7 +296,7 @@ def main():
     tuning = {name: hq._cppsim_amd.get_tuning(name) for name in ("block_swizzle", "tile_high_target")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
-    gates, step = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
+    gates, step, tiled_step = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
     n_pass = sum(1 for s in step if s[0] == "pass")
     n_perm = sum(1 for s in step if s[0] == "perm")
     plan_s = time.perf_counter() - t0
@@ -317,6 +334,31 @@ def main():
     per_launch_ms = tot / cnt if cnt else float("nan")
     achieved = 32.0 * 2 ** n / (per_launch_ms * 1e-3) / 1e9
     kernel_share = {name: round(ms, 2) for name, (c, ms) in times.items() if c}
+
+    # The same circuit with tile passes (the cluster scheduler targeting the
+    # LDS tile): fewer HBM passes, each applying several fused ops.
+    tile = None
+    if not args.no_tile:
+        for w in range(args.warmup):
+            run_step(sim, tiled_step, k)
+        sim.synchronize()
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        tp = 0
+        for s in range(args.steps):
+            tp = run_step(sim, tiled_step, k)
+        sim.synchronize()
+        el_t = time.perf_counter() - t0
+        tcnt, ttot = sim.kernel_time(hq.KERNEL_CLASSES.index("tile"))
+        sim.set_timing(False)
+        tile_ms = ttot / tcnt if tcnt else float("nan")
+        tile = {"hbm_passes_per_step": tp, "fused_passes_per_step": n_pass,
+                "ms_per_step": round(el_t / args.steps * 1e3, 2),
+                "speedup_vs_fused": round(ms_step / (el_t / args.steps * 1e3), 3),
+                "tile_pass_avg_ms": round(tile_ms, 4),
+                "tile_pass_hbm_gbs": round(32.0 * 2 ** n / (tile_ms * 1e-3) / 1e9, 1),
+                "op_width": k, "final_norm": sim.get_probability([], [])}
+        log("tile passes: %s" % tile)
     del sim
 
     sweep = None if args.no_sweep else kernel_sweep(hq, n)
@@ -357,6 +399,7 @@ def main():
                      "kernel": dom, "launches": cnt, "avg_launch_ms": round(per_launch_ms, 4)},
         "kernel_ms_total": kernel_share,
         "k_sweep_gbs": sweep,
+        "tile_planner": tile,
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,

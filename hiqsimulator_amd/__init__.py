@@ -29,7 +29,7 @@ Server = _cppsim_amd.Server
 
 # Kernel classes reported by SimulatorMPI.kernel_time(cls).
 KERNEL_CLASSES = ("dense_k1", "dense_k2", "dense_k3", "dense_k4", "dense_k5", "diag", "scale",
-                  "permute", "reduce", "normalize", "remove_bit", "fill_copy", "pauli")
+                  "permute", "reduce", "normalize", "remove_bit", "fill_copy", "pauli", "tile")
 
 __all__ = ["SimulatorMPI", "SwapScheduler", "ClusterScheduler", "Batch", "Server", "device_count",
            "KERNEL_CLASSES", "LIB_PATH"]

@@ -146,6 +146,29 @@ inline uint32_t sample_log_blocks(int n)
 hipError_t launch_sample(const void* arena, Prec prec, const SampleJob* jobs, int njobs,
                          double* sums, uint64_t* out, hipStream_t s);
 
+// ---- tile programs: many fused ops in one HBM pass --------------------------
+// A tile holds the 2^kTileBits amplitudes whose index bits outside
+// V = {0..5} + hpos are fixed; tile rank r numbers the V bits in ascending
+// order, so rank 0..5 are index bits 0..5 (64 contiguous amplitudes) and
+// rank 6 + j is index bit hpos[j].  One workgroup loads its tile into LDS,
+// applies every op in order (workgroup barrier between ops) and writes it
+// back: one read and one write of the state for the whole op list.
+constexpr int kTileBits = 12;
+constexpr int kTileHigh = kTileBits - 6;
+struct TileOp {
+     uint64_t mat_off;  // first complex entry of the op's 2^k x 2^k matrix
+     uint8_t k;
+     uint8_t rpos[5];  // ascending tile ranks; matrix bit l <-> rpos[l]
+     uint8_t pad[2];
+};
+struct TileHigh {
+     uint8_t hpos[kTileHigh];  // ascending index bits >= 6
+};
+// One tile pass over a state of 2^n amplitudes (n >= kTileBits): ops[0..nops)
+// (device array) with matrices in `mats` (device, state precision).
+hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& h, const TileOp* ops,
+                            int nops, int kmax, const void* mats, hipStream_t s);
+
 // ---- host side ------------------------------------------------------------
 struct BatchGate {
      std::vector<std::complex<double>> m;  // 2^t x 2^t, row-major, bit l <-> targets[l]

@@ -140,6 +140,66 @@ __global__ void __launch_bounds__(NT)
      }
 }
 
+// Tile program pass: workgroup b loads tile b (swizzled block order), runs
+// the op list on LDS and writes the tile back.  Chunk c (64 contiguous
+// amplitudes) of the tile sits at base + pdep(c, hpos).
+template <typename R, int KMAX, int NT>
+__global__ void __launch_bounds__(NT)
+    k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
+                const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz)
+{
+     using V = VT<R>;
+     constexpr int TS = 1 << kTileBits;
+     constexpr int NW = NT / 64;
+     constexpr int PER = (TS / 64) / NW;  // chunks per wave
+     __shared__ V tile[TS];
+     __shared__ V smat[1 << (2 * KMAX)];
+     const uint64_t b = swizzle_block(block_linear(), swz);
+     uint64_t base = b << 6;
+#pragma unroll
+     for (int j = 0; j < kTileHigh; ++j) {
+          const uint64_t lo = base & ((1ull << h.hpos[j]) - 1ull);
+          base = ((base ^ lo) << 1) | lo;
+     }
+     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+     V v[PER];
+#pragma unroll
+     for (int i = 0; i < PER; ++i) {
+          const int c = w + i * NW;
+          uint64_t off = 0;
+#pragma unroll
+          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << h.hpos[j];
+          v[i] = ld_nt(psi + base + off + lane);
+     }
+#pragma unroll
+     for (int i = 0; i < PER; ++i) tile[(w + i * NW) * 64 + lane] = v[i];
+     for (int p = 0; p < nops; ++p) {
+          const TileOp o = ops[p];
+          const uint32_t dd = 1u << (2 * o.k);
+          for (uint32_t i = threadIdx.x; i < dd; i += NT) smat[i] = mats[o.mat_off + i];
+          __syncthreads();
+          switch (o.k) {
+               case 1: wg_pass<1, R, NT>(tile, kTileBits, smat, o.rpos); break;
+               case 2: wg_pass<2, R, NT>(tile, kTileBits, smat, o.rpos); break;
+               case 3: wg_pass<3, R, NT>(tile, kTileBits, smat, o.rpos); break;
+               case 4: wg_pass<4, R, NT>(tile, kTileBits, smat, o.rpos); break;
+               default:
+                    if constexpr (KMAX >= 5) wg_pass<5, R, NT>(tile, kTileBits, smat, o.rpos);
+                    break;
+          }
+          __syncthreads();  // the op is complete before smat is refilled
+     }
+     __syncthreads();
+#pragma unroll
+     for (int i = 0; i < PER; ++i) {
+          const int c = w + i * NW;
+          uint64_t off = 0;
+#pragma unroll
+          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << h.hpos[j];
+          st_nt(psi + base + off + lane, tile[c * 64 + lane]);
+     }
+}
+
 template <int K, typename R>
 __global__ void __launch_bounds__(kBatchLayerBlock)
     k_batch_layer(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
@@ -422,6 +482,33 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 }  // namespace
+
+hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& h, const TileOp* ops,
+                            int nops, int kmax, const void* mats, hipStream_t s)
+{
+     if (n_qubits < kTileBits || n_qubits > 40 || nops < 0 || kmax < 1 || kmax > 5)
+          return hipErrorInvalidValue;
+     for (int j = 0; j < kTileHigh; ++j)
+          if (h.hpos[j] < 6 || h.hpos[j] >= n_qubits || (j && h.hpos[j] <= h.hpos[j - 1]))
+               return hipErrorInvalidValue;
+     const uint64_t blocks = 1ull << (n_qubits - kTileBits);
+     const int bits = get_block_swizzle();
+     const int swz = (bits > 0 && bits <= 16 && blocks % (1ull << bits) == 0) ? bits : 0;
+     const dim3 grid = grid_2d(blocks);
+#define HQ_TP(RR, KM, NT)                                                                        \
+     hipLaunchKernelGGL((k_tile_pass<RR, KM, NT>), grid, dim3(NT), 0, s, static_cast<VT<RR>*>(psi), \
+                        static_cast<const VT<RR>*>(mats), ops, nops, h, swz)
+     if (prec == Prec::C128) {
+          if (kmax >= 5) HQ_TP(double, 5, 512);
+          else HQ_TP(double, 4, 512);
+     }
+     else {
+          if (kmax >= 5) HQ_TP(float, 5, 512);
+          else HQ_TP(float, 4, 512);
+     }
+#undef HQ_TP
+     return hipGetLastError();
+}
 
 int batch_wg_threads(Prec prec, int kmax) { return prec == Prec::C64 && kmax <= 4 ? 1024 : 512; }
 

@@ -484,6 +484,51 @@ int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const 
      });
 }
 
+int hiqsim_apply_tiled(hiqsim_t* h, size_t ngates, const double* mats, const int64_t* tgt_ids,
+                       const size_t* tgt_off, const int64_t* ctrl_ids, const size_t* ctrl_off,
+                       int op_width, int* npasses)
+{
+     return guard([&] {
+          need(h, "handle");
+          auto bg = batch_gates(ngates, mats, tgt_ids, tgt_off, ctrl_ids, ctrl_off);
+          std::vector<hq::Engine::Gate> gates(bg.size());
+          for (size_t i = 0; i < bg.size(); ++i) {
+               const size_t dim = size_t(1) << bg[i].targets.size();
+               gates[i].m = hq::Mat(dim);
+               gates[i].m.a = std::move(bg[i].m);
+               gates[i].targets = std::move(bg[i].targets);
+               gates[i].ctrls = std::move(bg[i].ctrls);
+          }
+          const int np = h->eng.apply_tiled(gates, op_width);
+          if (npasses) *npasses = np;
+     });
+}
+
+int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, const int64_t* ctrl_pos,
+                            const size_t* ctrl_off, size_t ngates, int n_bits, int op_width,
+                            int32_t* pass_of, int32_t* op_of, int32_t* hpos, size_t* npasses)
+{
+     return guard([&] {
+          std::vector<uint8_t> diag(ngates, 0);
+          auto g = gate_list(gate_pos, gate_off, ctrl_pos, ctrl_off, diag.data(), ngates);
+          const auto plans = hq::plan_tile_passes(g, n_bits, hq::kTileHigh, op_width);
+          if (ngates) {
+               need(pass_of, "pass_of");
+               need(op_of, "op_of");
+               need(hpos, "hpos");
+          }
+          for (size_t p = 0; p < plans.size(); ++p) {
+               for (int j = 0; j < hq::kTileHigh; ++j) hpos[hq::kTileHigh * p + j] = plans[p].hpos[j];
+               for (size_t o = 0; o < plans[p].ops.size(); ++o)
+                    for (int gi: plans[p].ops[o]) {
+                         pass_of[gi] = static_cast<int32_t>(p);
+                         op_of[gi] = static_cast<int32_t>(o);
+                    }
+          }
+          if (npasses) *npasses = plans.size();
+     });
+}
+
 int hiqsim_set_tuning(const char* name, int64_t value)
 {
      return guard([&] {

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include "batch.hpp"
+#include "sched.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -104,6 +105,12 @@ Engine::~Engine()
      if (h_blocks_) (void)hipHostFree(h_blocks_);
      if (h_stage_) (void)hipHostFree(h_stage_);
      if (d_samples_) (void)hipFree(d_samples_);
+     if (prog_ev_) {
+          (void)hipEventSynchronize(prog_ev_);
+          (void)hipEventDestroy(prog_ev_);
+     }
+     if (h_prog_) (void)hipHostFree(h_prog_);
+     if (d_prog_) (void)hipFree(d_prog_);
      if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -754,6 +761,139 @@ void Engine::init_random_state(uint64_t seed)
      begin_timed(kScale);
      check(launch_scale(d_state_, prec_, n_amps_, f, 0.0, stream_), "scale");
      end_timed();
+}
+
+int Engine::apply_tiled(const std::vector<Gate>& gates, int op_width)
+{
+     DeviceScope ds(device_);
+     if (op_width < 1 || op_width > 5) throw std::runtime_error("apply_tiled(): op_width must be 1..5");
+     run();
+     if (gates.empty()) return 0;
+     const int n = static_cast<int>(n_qubits());
+     // bit positions of every gate
+     GateList gl;
+     for (const auto& g: gates) {
+          if (g.targets.empty() || g.targets.size() + g.ctrls.size() > 5)
+               throw std::runtime_error("apply_tiled(): a gate needs 1..5 qubits with its controls");
+          if (g.m.dim != (size_t(1) << g.targets.size()))
+               throw std::runtime_error("apply_tiled(): matrix size does not match targets");
+          std::vector<int64_t> t, c;
+          for (auto id: g.targets) t.push_back(static_cast<int64_t>(find_sure(id)));
+          for (auto id: g.ctrls) c.push_back(static_cast<int64_t>(find_sure(id)));
+          std::vector<int64_t> all = t;
+          all.insert(all.end(), c.begin(), c.end());
+          std::sort(all.begin(), all.end());
+          if (std::adjacent_find(all.begin(), all.end()) != all.end())
+               throw std::runtime_error("apply_tiled(): repeated qubit");
+          gl.gate.push_back(std::move(t));
+          gl.ctrl.push_back(std::move(c));
+          gl.diag.push_back(false);
+     }
+     auto fuse = [&](const std::vector<int>& idx, Mat& M, std::vector<int64_t>& pos) {
+          Fusion f;
+          for (int i: idx) {
+               Mat m = gates[i].m;
+               std::vector<int64_t> ids = gl.gate[i];
+               add_controls(m, ids, gl.ctrl[i]);
+               f.insert(std::move(m), false, std::move(ids), {});
+          }
+          std::vector<int64_t> ctrls;
+          bool diag = false;
+          f.perform(M, pos, ctrls, diag);
+     };
+     if (n < kTileBits) {
+          // small states: every op is one plain fused pass
+          const std::vector<int> cl = plan_clusters(gl, op_width, kPlanLookahead);
+          const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
+          std::vector<std::vector<int>> members(nc);
+          for (size_t i = 0; i < cl.size(); ++i) members[cl[i]].push_back(static_cast<int>(i));
+          for (const auto& mem: members) {
+               Mat M;
+               std::vector<int64_t> pos;
+               fuse(mem, M, pos);
+               std::vector<uint32_t> p32(pos.begin(), pos.end());
+               dense_or_diag(M, p32, 0, false);
+          }
+          return nc;
+     }
+     const std::vector<TilePlan> plans = plan_tile_passes(gl, n, kTileHigh, op_width);
+     // host image: per pass its ops, then all matrices
+     const size_t ab = amp_bytes(prec_);
+     std::vector<TileOp> ops;
+     std::vector<size_t> first;
+     std::vector<int> kmax;
+     std::vector<char> mats;
+     uint64_t entries = 0;
+     for (const auto& tp: plans) {
+          first.push_back(ops.size());
+          int km = 1;
+          int rank[64];
+          for (int p = 0; p < 64; ++p) rank[p] = -1;
+          for (int p = 0; p < 6; ++p) rank[p] = p;
+          for (int j = 0; j < kTileHigh; ++j) rank[tp.hpos[j]] = 6 + j;
+          for (const auto& op: tp.ops) {
+               Mat M;
+               std::vector<int64_t> pos;
+               fuse(op, M, pos);
+               TileOp to{};
+               to.k = static_cast<uint8_t>(pos.size());
+               if (to.k < 1 || to.k > 5) throw std::runtime_error("apply_tiled(): op width out of range");
+               for (size_t l = 0; l < pos.size(); ++l) {
+                    if (rank[pos[l]] < 0) throw std::runtime_error("apply_tiled(): op outside its tile");
+                    to.rpos[l] = static_cast<uint8_t>(rank[pos[l]]);
+               }
+               to.mat_off = entries;
+               km = std::max(km, static_cast<int>(to.k));
+               const size_t old = mats.size();
+               mats.resize(old + M.a.size() * ab);
+               for (size_t i = 0; i < M.a.size(); ++i) {
+                    if (prec_ == Prec::C128) {
+                         double* d = reinterpret_cast<double*>(mats.data() + old) + 2 * i;
+                         d[0] = M.a[i].real();
+                         d[1] = M.a[i].imag();
+                    }
+                    else {
+                         float* d = reinterpret_cast<float*>(mats.data() + old) + 2 * i;
+                         d[0] = static_cast<float>(M.a[i].real());
+                         d[1] = static_cast<float>(M.a[i].imag());
+                    }
+               }
+               entries += M.a.size();
+               ops.push_back(to);
+          }
+          kmax.push_back(km);
+     }
+     const size_t ops_bytes = (ops.size() * sizeof(TileOp) + 255) / 256 * 256;
+     const size_t need = ops_bytes + mats.size();
+     if (prog_busy_) check(hipEventSynchronize(prog_ev_), "hipEventSynchronize");
+     prog_busy_ = false;
+     if (need > prog_cap_) {
+          if (h_prog_) check(hipHostFree(h_prog_), "hipHostFree");
+          if (d_prog_) check(hipFree(d_prog_), "hipFree");
+          h_prog_ = d_prog_ = nullptr;
+          prog_cap_ = std::max(need, 2 * prog_cap_);
+          check(hipHostMalloc(reinterpret_cast<void**>(&h_prog_), prog_cap_, hipHostMallocDefault), "hipHostMalloc(program)");
+          check(hipMalloc(reinterpret_cast<void**>(&d_prog_), prog_cap_), "hipMalloc(program)");
+     }
+     if (!prog_ev_) check(hipEventCreateWithFlags(&prog_ev_, hipEventDisableTiming), "hipEventCreate");
+     std::memcpy(h_prog_, ops.data(), ops.size() * sizeof(TileOp));
+     std::memcpy(h_prog_ + ops_bytes, mats.data(), mats.size());
+     check(hipMemcpyAsync(d_prog_, h_prog_, need, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+     const TileOp* d_ops = reinterpret_cast<const TileOp*>(d_prog_);
+     const void* d_mats = d_prog_ + ops_bytes;
+     for (size_t p = 0; p < plans.size(); ++p) {
+          TileHigh th{};
+          for (int j = 0; j < kTileHigh; ++j) th.hpos[j] = static_cast<uint8_t>(plans[p].hpos[j]);
+          const size_t nops = plans[p].ops.size();
+          begin_timed(kTile);
+          check(launch_tile_pass(d_state_, prec_, n, th, d_ops + first[p], static_cast<int>(nops), kmax[p],
+                                 d_mats, stream_),
+                "tile pass");
+          end_timed();
+     }
+     check(hipEventRecord(prog_ev_, stream_), "hipEventRecord");
+     prog_busy_ = true;
+     return static_cast<int>(plans.size());
 }
 
 void Engine::sample(uint32_t shots, uint64_t seed, uint64_t* out)

@@ -37,6 +37,7 @@ enum KernelClass : int {
      kRemoveBit,
      kFillCopy,
      kPauli,
+     kTile,
      kNumKernelClasses
 };
 
@@ -105,6 +106,18 @@ public:
                       const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
      void apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
                           const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
+
+     // Applies a gate list (ids as in apply_gate; matrices 2^t x 2^t with t
+     // targets) with tile passes: each HBM pass loads tiles of index bits
+     // {0..5} + 6 planned bits into LDS and applies every op of its plan
+     // there (ops of at most op_width qubits, fused in FP64 on the host).
+     // Pending gates are run first.  Below 12 qubits the ops run as plain
+     // fused passes.  Returns the number of HBM passes.
+     struct Gate {
+          Mat m;
+          std::vector<int64_t> targets, ctrls;
+     };
+     int apply_tiled(const std::vector<Gate>& gates, int op_width);
 
      // One immediate pass at explicit bit positions (bypasses fusion).
      void apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
@@ -175,6 +188,11 @@ private:
      size_t stage_bytes_ = 0;
      uint64_t* d_samples_ = nullptr;  // sample outcomes
      size_t samples_cap_ = 0;
+     char* h_prog_ = nullptr;  // pinned staging of tile programs
+     char* d_prog_ = nullptr;
+     size_t prog_cap_ = 0;
+     hipEvent_t prog_ev_ = nullptr;
+     bool prog_busy_ = false;
 
      bool timing_ = false;
      std::vector<TimedLaunch> pending_;

@@ -67,6 +67,41 @@ void parse_terms(const py::list& terms, std::vector<uint64_t>& X, std::vector<ui
      }
 }
 
+using CArr = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>;
+
+// [(matrix, targets, controls), ...] in the CSR form of the batch entry points.
+struct GateCsr {
+     std::vector<double> mats;
+     std::vector<int64_t> tid, cid;
+     std::vector<size_t> toff{0}, coff{0};
+     size_t size() const { return toff.size() - 1; }
+};
+
+GateCsr gate_csr(const py::list& gates)
+{
+     GateCsr g;
+     for (auto item: gates) {
+          auto t = item.cast<py::sequence>();
+          if (py::len(t) != 3) throw std::runtime_error("gates must be (matrix, targets, controls)");
+          CArr m = t[0].cast<CArr>();
+          for (auto q: t[1].cast<py::sequence>()) g.tid.push_back(q.cast<int64_t>());
+          for (auto q: t[2].cast<py::sequence>()) g.cid.push_back(q.cast<int64_t>());
+          const size_t nt = g.tid.size() - g.toff.back();
+          if (nt == 0 || nt > 5) throw std::runtime_error("a gate needs 1..5 targets");
+          const size_t dd = (size_t(1) << nt) * (size_t(1) << nt);
+          if (static_cast<size_t>(m.size()) != dd)
+               throw std::runtime_error("gate matrix size does not match its targets");
+          const std::complex<double>* d = m.data();
+          for (size_t i = 0; i < dd; ++i) {
+               g.mats.push_back(d[i].real());
+               g.mats.push_back(d[i].imag());
+          }
+          g.toff.push_back(g.tid.size());
+          g.coff.push_back(g.cid.size());
+     }
+     return g;
+}
+
 class PySim {
 public:
      PySim(uint64_t seed, int64_t max_local, int64_t max_cluster, const std::string& precision,
@@ -310,6 +345,15 @@ public:
           py::gil_scoped_release r;
           ok(hiqsim_init_random_state(h_, seed));
      }
+     int apply_tiled(const py::list& gates, int op_width)
+     {
+          const GateCsr g = gate_csr(gates);
+          int np = 0;
+          py::gil_scoped_release r;
+          ok(hiqsim_apply_tiled(h_, g.size(), g.mats.data(), g.tid.data(), g.toff.data(), g.cid.data(),
+                                g.coff.data(), op_width, &np));
+          return np;
+     }
      py::array_t<uint64_t> sample(uint32_t shots, uint64_t seed)
      {
           py::array_t<uint64_t> a(static_cast<py::ssize_t>(shots));
@@ -421,41 +465,6 @@ private:
      int locals_;
      bool fuse_;
 };
-
-using CArr = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>;
-
-// [(matrix, targets, controls), ...] in the CSR form of the batch entry points.
-struct GateCsr {
-     std::vector<double> mats;
-     std::vector<int64_t> tid, cid;
-     std::vector<size_t> toff{0}, coff{0};
-     size_t size() const { return toff.size() - 1; }
-};
-
-GateCsr gate_csr(const py::list& gates)
-{
-     GateCsr g;
-     for (auto item: gates) {
-          auto t = item.cast<py::sequence>();
-          if (py::len(t) != 3) throw std::runtime_error("gates must be (matrix, targets, controls)");
-          CArr m = t[0].cast<CArr>();
-          for (auto q: t[1].cast<py::sequence>()) g.tid.push_back(q.cast<int64_t>());
-          for (auto q: t[2].cast<py::sequence>()) g.cid.push_back(q.cast<int64_t>());
-          const size_t nt = g.tid.size() - g.toff.back();
-          if (nt == 0 || nt > 5) throw std::runtime_error("a gate needs 1..5 targets");
-          const size_t dd = (size_t(1) << nt) * (size_t(1) << nt);
-          if (static_cast<size_t>(m.size()) != dd)
-               throw std::runtime_error("gate matrix size does not match its targets");
-          const std::complex<double>* d = m.data();
-          for (size_t i = 0; i < dd; ++i) {
-               g.mats.push_back(d[i].real());
-               g.mats.push_back(d[i].imag());
-          }
-          g.toff.push_back(g.tid.size());
-          g.coff.push_back(g.cid.size());
-     }
-     return g;
-}
 
 // Many independent circuits run together (hiqsim_batch_*).
 class PyBatch {
@@ -706,6 +715,7 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("compare_snapshot", &PySim::compare_snapshot)
          .def("init_random_state", &PySim::init_random_state)
          .def("sample", &PySim::sample, py::arg("shots"), py::arg("seed"))
+         .def("apply_tiled", &PySim::apply_tiled, py::arg("gates"), py::arg("op_width") = 4)
          .def("restore", &PySim::restore)
          .def("synchronize", &PySim::synchronize)
          .def("set_timing", &PySim::set_timing)
@@ -717,6 +727,22 @@ PYBIND11_MODULE(_cppsim_amd, m)
           ok(hiqsim_get_tuning(name.c_str(), &v));
           return v;
      });
+     m.def("plan_tile_passes",
+           [](const std::vector<std::vector<int64_t>>& gate,
+              const std::vector<std::vector<int64_t>>& ctrl, int n_bits, int op_width) {
+                CsrGates g(gate, ctrl, std::vector<bool>(gate.size(), false));
+                std::vector<int32_t> pass_of(gate.size()), op_of(gate.size()), hpos(6 * gate.size() + 6);
+                size_t np = 0;
+                {
+                     py::gil_scoped_release r;
+                     ok(hiqsim_plan_tile_passes(g.gid.data(), g.goff.data(), g.cid.data(), g.coff.data(),
+                                                gate.size(), n_bits, op_width, pass_of.data(), op_of.data(),
+                                                hpos.data(), &np));
+                }
+                hpos.resize(6 * np);
+                return py::make_tuple(pass_of, op_of, hpos);
+           },
+           py::arg("gate"), py::arg("gate_ctrl"), py::arg("n_bits"), py::arg("op_width") = 4);
      m.def("plan_clusters",
            [](const std::vector<std::vector<int64_t>>& gate,
               const std::vector<std::vector<int64_t>>& ctrl, int cluster_size, int mode) {

@@ -509,6 +509,114 @@ std::vector<int> plan_clusters(const GateList& g, int cluster_size, int mode)
      return cluster_of;
 }
 
+std::vector<TilePlan> plan_tile_passes(const GateList& g, int n, int tile_high, int op_width)
+{
+     check_sizes(g);
+     if (n < 6 + tile_high || n > 63) throw std::runtime_error("plan_tile_passes(): need 6 + tile_high <= n <= 63");
+     if (tile_high < 1 || op_width < 1) throw std::runtime_error("plan_tile_passes(): bad tile or op width");
+     const size_t ng = g.gate.size();
+     std::vector<msk_t> q(ng);
+     for (size_t i = 0; i < ng; ++i) {
+          msk_t m = 0;
+          for (auto p: g.gate[i]) m |= msk_t(1) << p;
+          for (auto p: g.ctrl[i]) m |= msk_t(1) << p;
+          for (auto p: g.gate[i])
+               if (p < 0 || p >= n) throw std::runtime_error("plan_tile_passes(): position out of range");
+          for (auto p: g.ctrl[i])
+               if (p < 0 || p >= n) throw std::runtime_error("plan_tile_passes(): position out of range");
+          if (popc(m & ~msk_t(63)) > tile_high) throw std::runtime_error("plan_tile_passes(): a gate does not fit the tile");
+          q[i] = m;
+     }
+     const msk_t kFree = 63;
+     std::vector<size_t> remaining(ng);
+     for (size_t i = 0; i < ng; ++i) remaining[i] = i;
+     auto score = [&](msk_t S) {
+          msk_t blocked = 0;
+          int cnt = 0;
+          for (size_t j = 0; j < remaining.size(); ++j) {
+               const msk_t m = q[remaining[j]];
+               if (!(m & blocked) && submask(m, S)) ++cnt;
+               else blocked |= m;
+               if (submask(S, blocked)) break;
+          }
+          return cnt;
+     };
+     const size_t kWindow = 64;
+     std::vector<TilePlan> out;
+     while (!remaining.empty()) {
+          // pool: high positions of the leading gates, in order of first use
+          std::vector<int> pool;
+          msk_t seen = 0;
+          for (size_t j = 0; j < remaining.size() && j < kWindow; ++j)
+               for (msk_t m = q[remaining[j]] & ~kFree & ~seen; m; m &= m - 1) {
+                    pool.push_back(__builtin_ctzll(m));
+                    seen |= m & (~m + 1);
+               }
+          msk_t H = q[remaining[0]] & ~kFree;  // the first gate always fits
+          int best = score(kFree | H);
+          while (popc(H) < tile_high) {
+               int add = -1, add_s = -1;
+               for (int p: pool) {
+                    if ((H >> p) & 1) continue;
+                    const int s = score(kFree | H | (msk_t(1) << p));
+                    if (s > add_s) {
+                         add_s = s;
+                         add = p;
+                    }
+               }
+               if (add < 0) break;
+               H |= msk_t(1) << add;
+               best = add_s;
+          }
+          // one round of single swaps
+          for (msk_t out_b = H; out_b; out_b &= out_b - 1) {
+               const msk_t ob = out_b & (~out_b + 1);
+               for (int p: pool) {
+                    if ((H >> p) & 1) continue;
+                    const msk_t H2 = (H & ~ob) | (msk_t(1) << p);
+                    const int s = score(kFree | H2);
+                    if (s > best) {
+                         best = s;
+                         H = H2;
+                         break;
+                    }
+               }
+          }
+          const msk_t S = kFree | H;
+          // taken gates
+          msk_t blocked = 0;
+          std::vector<size_t> taken, next;
+          for (size_t j = 0; j < remaining.size(); ++j) {
+               const msk_t m = q[remaining[j]];
+               if (!(m & blocked) && submask(m, S)) taken.push_back(remaining[j]);
+               else {
+                    blocked |= m;
+                    next.push_back(remaining[j]);
+               }
+          }
+          if (taken.empty()) throw std::runtime_error("plan_tile_passes(): no gate could be scheduled");
+          remaining.swap(next);
+          // pad the high positions to tile_high with the lowest free ones
+          for (int p = 6; p < n && popc(H) < tile_high; ++p) H |= msk_t(1) << p;
+          TilePlan tp;
+          for (int p = 6; p < n; ++p)
+               if ((H >> p) & 1) tp.hpos.push_back(p);
+          // ops: lookahead packing of the taken gates
+          GateList sub;
+          for (size_t i: taken) {
+               sub.gate.push_back(g.gate[i]);
+               sub.ctrl.push_back(g.ctrl[i]);
+               sub.diag.push_back(false);
+          }
+          const std::vector<int> cl = plan_clusters(sub, op_width, kPlanLookahead);
+          const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
+          tp.ops.assign(nc, {});
+          for (size_t j = 0; j < taken.size(); ++j) tp.ops[cl[j]].push_back(static_cast<int>(taken[j]));
+          out.push_back(std::move(tp));
+     }
+     return out;
+}
+
 std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals, bool fuse)
 {
      Swap s(g, num_splits, num_locals, fuse);

@@ -49,6 +49,20 @@ std::vector<int> schedule_cluster(const GateList& g, const std::vector<int64_t>&
 enum PlanMode : int { kPlanReference = 0, kPlanLookahead = 1 };
 std::vector<int> plan_clusters(const GateList& g, int cluster_size, int mode);
 
+// Tile-capacity planning (the cluster scheduler retargeted from MPI rank
+// locality to the LDS tile of one workgroup): ids are index bit positions.
+// Each pass holds the gates whose bits all lie in {0..5} + hpos, where hpos
+// are `tile_high` positions >= 6 chosen to take as many gates as possible
+// from the front of the remaining list (same commutation rule as
+// ClusterScheduler: a gate is taken if it touches no bit of a gate skipped
+// before it).  The taken gates are cut into ops of at most op_width bits
+// (lookahead packing); every op becomes one fused matrix applied in LDS.
+struct TilePlan {
+     std::vector<int> hpos;              // tile_high ascending positions in [6, n)
+     std::vector<std::vector<int>> ops;  // gate indices of each op, program order
+};
+std::vector<TilePlan> plan_tile_passes(const GateList& g, int n, int tile_high, int op_width);
+
 // swap_scheduler.cpp:70-88.  Returns the qubit ids to be local.
 std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals,
                                    bool fuse);

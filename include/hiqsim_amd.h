@@ -183,6 +183,25 @@ int hiqsim_plan_clusters(const int64_t* gate_ids, const size_t* gate_off, const 
                          const size_t* ctrl_off, size_t ngates, int cluster_size, int mode,
                          int32_t* cluster_of);
 
+/* Applies a gate list with tile passes (no reference counterpart; the
+ * reference applies one fused cluster per state pass, SimulatorMPI::Run,
+ * SimulatorMPI.cpp:441-541).  Gates as in hiqsim_batch_add_circuit but on
+ * the handle's qubit ids.  Each HBM pass loads tiles of index bits {0..5}
+ * plus 6 planned bits into LDS and applies there every op its plan holds
+ * (ops of at most op_width qubits, fused in FP64); the plan is the cluster
+ * scheduler's rule with the tile as the local set.  Pending gates are run
+ * first.  *npasses = HBM passes used. */
+int hiqsim_apply_tiled(hiqsim_t* h, size_t ngates, const double* mats, const int64_t* tgt_ids,
+                       const size_t* tgt_off, const int64_t* ctrl_ids, const size_t* ctrl_off,
+                       int op_width, int* npasses);
+/* The tile plan alone (no device): gates given by bit positions (CSR) on
+ * n_bits bits.  pass_of[g], op_of[g] = pass and op (execution order) of
+ * gate g; hpos[6 * p .. 6 * p + 6) = the planned bits of pass p;
+ * *npasses set (hpos must hold 6 * ngates entries). */
+int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, const int64_t* ctrl_pos,
+                            const size_t* ctrl_off, size_t ngates, int n_bits, int op_width,
+                            int32_t* pass_of, int32_t* op_of, int32_t* hpos, size_t* npasses);
+
 /* Process-wide kernel tuning (no reference counterpart):
  *   "tile_high_target": dense passes whose targets all sit at or above this
  *                       bit take the direct form (default 20);
