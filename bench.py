@@ -187,6 +187,7 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
     log_fn("batch: %d circuits generated (%.1f s)" % (count, time.perf_counter() - t0))
     out = {"workload": "%d circuits, n~U{10..22}, depth~U{10..60}, Haar 2q every 5th layer"
                        % count, "gates": int(sum(len(g) for _, g in circuits))}
+    # effective_gbs counts 2 * B * 2^n bytes per HBM pass (fused or tile program)
     sample_idx = list(range(0, count, max(1, count // 64)))[:64]
     states = {}
     for prec in ("c64", "c128"):
@@ -194,8 +195,10 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
         k_by_n = tune["k_by_n"] if tune else {}
         wg_max = tune["wg_max_qubits"] if tune else 16
         tile_min = tune["tile_min_qubits"] if tune else 0
+        prog_min = tune["prog_min_qubits"] if tune else 31
         t0 = time.perf_counter()
-        b = make_batch(circuits, prec, k_by_n, wg_max, tile_min_qubits=tile_min)
+        b = make_batch(circuits, prec, k_by_n, wg_max, tile_min_qubits=tile_min,
+                       prog_min_qubits=prog_min)
         build_s = time.perf_counter() - t0
         b.run()
         ms = min(b.run() for _ in range(3))
@@ -208,7 +211,9 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
                      "lds_circuits": st["lds_circuits"], "layer_launches": st["layer_launches"],
                      "arena_gib": round(st["arena_bytes"] / 2 ** 30, 2), "build_s": round(build_s, 2),
                      "plan": "autotuned table" if tune else "default (k=4, wg_max_qubits=16)",
-                     "wg_max_qubits": wg_max, "tile_min_qubits": tile_min}
+                     "wg_max_qubits": wg_max, "tile_min_qubits": tile_min,
+                     "prog_min_qubits": prog_min, "tile_program_passes": st["tile_passes"],
+                     "tile_program_ops": st["tile_ops"]}
         log_fn("batch %s: %s" % (prec, out[prec]))
         del b
     fids = [fidelity(a, b) for a, b in zip(states["c128"], states["c64"])]

@@ -12,7 +12,11 @@ knobs, both picked here by measurement:
   per state;
 * ``tile_min_qubits``: layered passes of circuits from this many qubits on
   may take the wave-tile form (LDS regrouping, non-temporal streaming);
-  smaller ones use the direct form.
+  smaller ones use the direct form;
+* ``prog_min_qubits``: layered circuits from this many qubits on (at least
+  12) run as tile programs: each HBM pass loads 4096-amplitude tiles into
+  LDS and applies several fused ops there (the cluster scheduler targeting
+  the tile), with k_by_n[n] as the op width.
 """
 import json
 import os
@@ -40,6 +44,7 @@ def load_tuning(precision, arch="gfx950"):
         t = json.load(f)
     t["k_by_n"] = {int(n): int(k) for n, k in t["k_by_n"].items()}
     t.setdefault("tile_min_qubits", 0)
+    t.setdefault("prog_min_qubits", 31)
     return t
 
 
@@ -49,6 +54,7 @@ def save_tuning(result, precision, arch="gfx950"):
         json.dump({"k_by_n": {str(n): k for n, k in sorted(result["k_by_n"].items())},
                    "wg_max_qubits": result["wg_max_qubits"],
                    "tile_min_qubits": result.get("tile_min_qubits", 0),
+                   "prog_min_qubits": result.get("prog_min_qubits", 31),
                    "probe_ms": result.get("probe_ms", {})}, f, indent=1, sort_keys=True)
 
 
@@ -65,12 +71,12 @@ def batch_workload(count, seed, n_range=(10, 22), depth_range=(10, 60), haar_eve
 
 
 def make_batch(circuits, precision="c128", k_by_n=None, wg_max_qubits=DEFAULT_WG_MAX_QUBITS,
-               device=0, build=True, tile_min_qubits=0):
+               device=0, build=True, tile_min_qubits=0, prog_min_qubits=31):
     """A ``Batch`` holding ``circuits`` ([(n, gates), ...]), planned and built."""
     b = Batch(precision, device)
     for n, gates in circuits:
         b.add_circuit(n, gates)
-    b.set_plan(dict(k_by_n or {}), int(wg_max_qubits), int(tile_min_qubits))
+    b.set_plan(dict(k_by_n or {}), int(wg_max_qubits), int(tile_min_qubits), int(prog_min_qubits))
     if build:
         b.build()
     return b
@@ -89,12 +95,13 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
     random circuit (enough copies to hold about ``amp_budget`` amplitudes,
     8..2048) is timed for every width in ``widths``, on the persistent workgroup
     path (n <= ``wg_limit``) and on the layered path (n >= ``layer_floor``)
-    in its direct and wave-tile forms.
+    as direct passes, wave-tile passes and (n >= 12) tile programs.
     Returns ``{"k_by_n": {n: k}, "wg_max_qubits": t, "tile_min_qubits": m,
-    "probe_ms": {...}}`` where t is the largest n whose best workgroup time
-    beats its best layered time (the split is monotone in n by
-    construction) and m the smallest n from which the tile form wins for
-    every larger layered n.
+    "prog_min_qubits": p, "probe_ms": {...}}`` where t is the largest n whose
+    best workgroup time beats its best layered time (the split is monotone
+    in n by construction), p the smallest n from which tile programs win
+    for every larger layered n, and m the same for the wave-tile form among
+    the remaining layered n.
     """
     ns = sorted(set(int(n) for n in ns))
     best = {}
@@ -104,15 +111,18 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
         b = Batch(precision, device)
         b.add_circuit(n, random_layers_circuit(n, depth, seed + n, haar_every=5))
         b.add_copies(0, copies - 1)
-        for path, limit, tmin in (("wg", 24, 31), ("layer", 0, 31), ("tile", 0, 0)):
+        for path, limit, tmin, pmin in (("wg", 24, 31, 31), ("layer", 0, 31, 31), ("tile", 0, 0, 31),
+                                        ("prog", 0, 31, 0)):
             if path == "wg" and n > wg_limit:
                 continue
             if path != "wg" and n < layer_floor:
                 continue
+            if path == "prog" and n < 12:
+                continue
             for k in widths:
                 if k > n:
                     continue
-                b.set_plan({n: k}, limit, tmin)
+                b.set_plan({n: k}, limit, tmin, pmin)
                 b.build()
                 ms = _time_batch(b, reps)
                 probe[(n, path, k)] = ms
@@ -121,8 +131,8 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
                 if (n, path) not in best or ms < best[(n, path)][1]:
                     best[(n, path)] = (k, ms)
         del b
-    def layered(n):
-        cands = [best[(n, p)] + (p,) for p in ("layer", "tile") if (n, p) in best]
+    def layered(n, forms=("layer", "tile", "prog")):
+        cands = [best[(n, p)] + (p,) for p in forms if (n, p) in best]
         return min(cands, key=lambda c: c[1]) if cands else None
 
     wg_max = 0
@@ -133,9 +143,16 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
             wg_max = n
         else:
             break
-    tile_min = 31
+    prog_min = 31
     for n in reversed([n for n in ns if n > wg_max]):
         ly = layered(n)
+        if ly is not None and ly[2] == "prog":
+            prog_min = n
+        else:
+            break
+    tile_min = 31
+    for n in reversed([n for n in ns if wg_max < n < prog_min]):
+        ly = layered(n, ("layer", "tile"))
         if ly is not None and ly[2] == "tile":
             tile_min = n
         else:
@@ -143,11 +160,14 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
     k_by_n = {}
     for n in ns:
         if n <= wg_max:
-            k_by_n[n] = best[(n, "wg")][0] if (n, "wg") in best else 4
+            form = "wg"
+        elif n >= prog_min:
+            form = "prog"
         else:
             form = "tile" if n >= tile_min else "layer"
-            k_by_n[n] = best[(n, form)][0] if (n, form) in best else 4
+        k_by_n[n] = best[(n, form)][0] if (n, form) in best else 4
     return {"k_by_n": k_by_n, "wg_max_qubits": wg_max, "tile_min_qubits": tile_min,
+            "prog_min_qubits": prog_min,
             "probe_ms": {"%d/%s/k%d" % key: round(v, 4) for key, v in probe.items()}}
 
 

@@ -83,6 +83,61 @@ std::vector<PlannedPass> plan_circuit(const std::vector<BatchGate>& gates, int k
      return out;
 }
 
+// One tile pass of a circuit: its planned bits and its fused ops, whose
+// positions are tile ranks (0..5 = index bits 0..5, 6 + j = hpos[j]).
+struct PlannedTile {
+     uint8_t hpos[kTileHigh];
+     std::vector<PlannedPass> ops;
+};
+
+std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width)
+{
+     std::vector<PlannedTile> out;
+     if (gates.empty()) return out;
+     GateList gl;
+     for (const auto& g: gates) {
+          gl.gate.push_back(g.targets);
+          gl.ctrl.push_back(g.ctrls);
+          gl.diag.push_back(false);
+     }
+     for (const TilePlan& tp: plan_tile_passes(gl, n, kTileHigh, op_width)) {
+          PlannedTile t;
+          int rank[64];
+          for (int p = 0; p < 64; ++p) rank[p] = p < 6 ? p : -1;
+          for (int j = 0; j < kTileHigh; ++j) {
+               t.hpos[j] = static_cast<uint8_t>(tp.hpos[j]);
+               rank[tp.hpos[j]] = 6 + j;
+          }
+          for (const auto& op: tp.ops) {
+               Fusion f;
+               for (int gi: op) {
+                    const BatchGate& g = gates[gi];
+                    Mat m(size_t(1) << g.targets.size());
+                    m.a = g.m;
+                    std::vector<int64_t> ids = g.targets;
+                    add_controls(m, ids, g.ctrls);
+                    f.insert(std::move(m), false, std::move(ids), {});
+               }
+               Mat M;
+               std::vector<int64_t> ids, ctrls;
+               bool diag = false;
+               f.perform(M, ids, ctrls, diag);
+               if (ids.size() > static_cast<size_t>(kMaxWidth) || !ctrls.empty())
+                    throw std::runtime_error("batch: a fused op spans more than 5 qubits");
+               PlannedPass p;
+               p.k = static_cast<int>(ids.size());
+               for (int l = 0; l < p.k; ++l) {
+                    if (rank[ids[l]] < 0) throw std::runtime_error("batch: op outside its tile");
+                    p.pos[l] = static_cast<uint8_t>(rank[ids[l]]);
+               }
+               p.m = std::move(M.a);
+               t.ops.push_back(std::move(p));
+          }
+          out.push_back(std::move(t));
+     }
+     return out;
+}
+
 template <class F>
 void parallel_for(size_t n, F&& f)
 {
@@ -196,10 +251,13 @@ int64_t Batch::add_copies(int64_t c, int64_t count)
      return first;
 }
 
-void Batch::set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits)
+void Batch::set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits,
+                     int prog_min_qubits)
 {
      if (tile_min_qubits < 0 || tile_min_qubits > 31)
           throw std::runtime_error("batch: tile_min_qubits must be 0..31");
+     if (prog_min_qubits < 0 || prog_min_qubits > 31)
+          throw std::runtime_error("batch: prog_min_qubits must be 0..31");
      for (int k: k_by_n)
           if (k < 0 || k > kMaxWidth) throw std::runtime_error("batch: fusion width must be 0..5");
      if (wg_max_qubits < 0 || wg_max_qubits > kMaxWgQubits)
@@ -207,6 +265,7 @@ void Batch::set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile
      k_by_n_ = k_by_n;
      wg_max_qubits_ = wg_max_qubits;
      tile_min_qubits_ = tile_min_qubits;
+     prog_min_qubits_ = prog_min_qubits;
      built_ = false;
 }
 
@@ -233,11 +292,17 @@ Batch::HostPlan Batch::plan_host(int workgroups)
      if (nc == 0) throw std::runtime_error("batch: no circuits");
      const size_t ab = amp_bytes(prec_);
 
-     std::vector<std::vector<PlannedPass>> plan(nc);
-     parallel_for(nc, [&](size_t c) { plan[c] = plan_circuit(circ_[c].gates, fusion_width(circ_[c].n)); });
-
      std::vector<size_t> layer_c, wg_c;
      for (size_t c = 0; c < nc; ++c) (circ_[c].n > wg_max_qubits_ ? layer_c : wg_c).push_back(c);
+     const int prog_min = std::max(prog_min_qubits_, kTileBits);
+     std::vector<uint8_t> is_prog(nc, 0);
+     for (size_t c: layer_c) is_prog[c] = circ_[c].n >= prog_min;
+     std::vector<std::vector<PlannedPass>> plan(nc);
+     std::vector<std::vector<PlannedTile>> tiles(nc);
+     parallel_for(nc, [&](size_t c) {
+          if (is_prog[c]) tiles[c] = plan_circuit_tiles(circ_[c].gates, circ_[c].n, fusion_width(circ_[c].n));
+          else plan[c] = plan_circuit(circ_[c].gates, fusion_width(circ_[c].n));
+     });
      uint64_t off = 0;
      for (size_t c: layer_c) {
           circ_[c].state_off = off;
@@ -250,8 +315,9 @@ Batch::HostPlan Batch::plan_host(int workgroups)
      }
      hp.arena_amps = off;
 
-     // Matrices, concatenated in the state's precision.
-     std::vector<std::vector<uint64_t>> mat_off(nc);
+     // Matrices, concatenated in the state's precision: each circuit's fused
+     // passes, then its tile ops.
+     std::vector<std::vector<uint64_t>> mat_off(nc), tmat_off(nc);
      uint64_t mat_entries = 0;
      for (size_t c = 0; c < nc; ++c) {
           mat_off[c].resize(plan[c].size());
@@ -262,22 +328,39 @@ Batch::HostPlan Batch::plan_host(int workgroups)
                stats_.kmax = std::max(stats_.kmax, plan[c][p].k);
           }
           stats_.passes += plan[c].size();
+          for (const auto& t: tiles[c]) {
+               for (const auto& op: t.ops) {
+                    tmat_off[c].push_back(mat_entries);
+                    mat_entries += op.m.size();
+                    stats_.kmax = std::max(stats_.kmax, op.k);
+               }
+               stats_.amp_passes += uint64_t(1) << circ_[c].n;
+               stats_.tile_ops += t.ops.size();
+          }
+          stats_.passes += tiles[c].size();
+          stats_.tile_passes += tiles[c].size();
      }
      hp.mat_entries = mat_entries;
      hp.mats.resize(std::max<uint64_t>(mat_entries, 1) * ab);
      std::vector<char>& h_mats = hp.mats;
      parallel_for(nc, [&](size_t c) {
-          for (size_t p = 0; p < plan[c].size(); ++p) {
-               const auto& m = plan[c][p].m;
+          std::vector<std::pair<const std::vector<cplx>*, uint64_t>> all;
+          for (size_t p = 0; p < plan[c].size(); ++p) all.push_back({&plan[c][p].m, mat_off[c][p]});
+          size_t q = 0;
+          for (const auto& t: tiles[c])
+               for (const auto& op: t.ops) all.push_back({&op.m, tmat_off[c][q++]});
+          for (const auto& mo: all) {
+               const auto& m = *mo.first;
+               const uint64_t moff = mo.second;
                if (prec_ == Prec::C128) {
-                    double* d = reinterpret_cast<double*>(h_mats.data()) + 2 * mat_off[c][p];
+                    double* d = reinterpret_cast<double*>(h_mats.data()) + 2 * moff;
                     for (size_t i = 0; i < m.size(); ++i) {
                          d[2 * i] = m[i].real();
                          d[2 * i + 1] = m[i].imag();
                     }
                }
                else {
-                    float* d = reinterpret_cast<float*>(h_mats.data()) + 2 * mat_off[c][p];
+                    float* d = reinterpret_cast<float*>(h_mats.data()) + 2 * moff;
                     for (size_t i = 0; i < m.size(); ++i) {
                          d[2 * i] = static_cast<float>(m[i].real());
                          d[2 * i + 1] = static_cast<float>(m[i].imag());
@@ -348,8 +431,8 @@ Batch::HostPlan Batch::plan_host(int workgroups)
      for (size_t c: layer_c) max_len = std::max(max_len, plan[c].size());
      for (size_t L = 0; L < max_len; ++L) {
           for (int k = 1; k <= kMaxWidth; ++k) {
-               LayerLaunch ld{k, false, h_items.size(), 0, 0};
-               LayerLaunch lt{k, true, h_titems.size(), 0, 0};
+               LayerLaunch ld{k, kDirect, h_items.size(), 0, 0};
+               LayerLaunch lt{k, kTileForm, h_titems.size(), 0, 0};
                for (size_t c: layer_c) {
                     if (L >= plan[c].size() || plan[c][L].k != k) continue;
                     const PlannedPass& pp = plan[c][L];
@@ -382,6 +465,39 @@ Batch::HostPlan Batch::plan_host(int workgroups)
                if (ld.count) launches.push_back(ld);
                if (lt.count) launches.push_back(lt);
           }
+     }
+     // Tile-program circuits: launch L holds the L-th tile pass of each.
+     size_t max_tiles = 0;
+     for (size_t c: layer_c) max_tiles = std::max(max_tiles, tiles[c].size());
+     for (size_t L = 0; L < max_tiles; ++L) {
+          LayerLaunch lp{1, kTileProgram, hp.progs.size(), 0, 0};
+          for (size_t c: layer_c) {
+               if (!is_prog[c] || L >= tiles[c].size()) continue;
+               const PlannedTile& t = tiles[c][L];
+               size_t q0 = 0;
+               for (size_t l = 0; l < L; ++l) q0 += tiles[c][l].ops.size();
+               BatchTileProg it{};
+               it.state_off = circ_[c].state_off;
+               it.block_start = lp.blocks;
+               it.op_first = static_cast<uint32_t>(hp.tops.size());
+               it.nops = static_cast<uint16_t>(t.ops.size());
+               it.log_blocks = static_cast<uint8_t>(circ_[c].n - kTileBits);
+               std::memcpy(it.hpos, t.hpos, kTileHigh);
+               for (size_t o = 0; o < t.ops.size(); ++o) {
+                    TileOp to{};
+                    to.mat_off = tmat_off[c][q0 + o];
+                    to.k = static_cast<uint8_t>(t.ops[o].k);
+                    std::memcpy(to.rpos, t.ops[o].pos, 5);
+                    lp.k = std::max(lp.k, t.ops[o].k);
+                    hp.tops.push_back(to);
+               }
+               if (t.ops.size() > 65535) throw std::runtime_error("batch: too many ops in a tile pass");
+               lp.blocks += uint64_t(1) << it.log_blocks;
+               hp.progs.push_back(it);
+               hp.prog_n.push_back(circ_[c].n);
+               ++lp.count;
+          }
+          if (lp.count) launches.push_back(lp);
      }
      stats_.layer_launches = launches.size();
      for (const auto& ll: launches)
@@ -446,7 +562,26 @@ void Batch::check_plan(const HostPlan& hp) const
      for (const LayerLaunch& ll: hp.launches) {
           uint64_t next = 0;
           for (size_t i = ll.first; i < ll.first + ll.count; ++i) {
-               if (ll.tile) {
+               if (ll.kind == kTileProgram) {
+                    if (i >= hp.progs.size()) plan_error("tile program index");
+                    const BatchTileProg& it = hp.progs[i];
+                    const int n = hp.prog_n[i];
+                    if (n < kTileBits || it.log_blocks != n - kTileBits || it.block_start != next)
+                         plan_error("tile program blocks");
+                    next += uint64_t(1) << it.log_blocks;
+                    if (!state_ok(it.state_off, n)) plan_error("tile program state");
+                    for (int j = 0; j < kTileHigh; ++j)
+                         if (it.hpos[j] < 6 || it.hpos[j] >= n || (j && it.hpos[j] <= it.hpos[j - 1]))
+                              plan_error("tile program bits");
+                    if (static_cast<size_t>(it.op_first) + it.nops > hp.tops.size()) plan_error("tile program ops");
+                    for (size_t o = it.op_first; o < it.op_first + it.nops; ++o) {
+                         const TileOp& to = hp.tops[o];
+                         if (to.k > ll.k || !mat_ok(to.mat_off, to.k) || !pos_ok(to.rpos, to.k, kTileBits))
+                              plan_error("tile program op");
+                    }
+                    continue;
+               }
+               if (ll.kind == kTileForm) {
                     if (i >= hp.titems.size()) plan_error("tile item index");
                     const BatchTileItem& it = hp.titems[i];
                     const int n = hp.titem_n[i];
@@ -515,7 +650,10 @@ void Batch::build()
      const size_t sz_items = align_up(hp.items.size() * sizeof(BatchItem), 256);
      const size_t sz_titems = align_up(hp.titems.size() * sizeof(BatchTileItem), 256);
      const size_t sz_ones = align_up(hp.ones.size() * sizeof(uint64_t), 256);
-     const size_t desc_bytes = sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_titems + sz_ones + 256;
+     const size_t sz_progs = align_up(hp.progs.size() * sizeof(BatchTileProg), 256);
+     const size_t sz_tops = align_up(hp.tops.size() * sizeof(TileOp), 256);
+     const size_t desc_bytes = sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_titems + sz_ones + sz_progs +
+                               sz_tops + 256;
      const size_t need = stats_.arena_bytes + hp.mats.size() + desc_bytes;
      if (need + (size_t(1) << 30) > free_b) {
           std::ostringstream os;
@@ -534,6 +672,8 @@ void Batch::build()
      auto* d_items = reinterpret_cast<BatchItem*>(base + sz_pass + sz_circ + sz_woff + sz_wl);
      auto* d_titems = reinterpret_cast<BatchTileItem*>(base + sz_pass + sz_circ + sz_woff + sz_wl + sz_items);
      auto* d_ones = reinterpret_cast<uint64_t*>(base + sz_pass + sz_circ + sz_woff + sz_wl + sz_items + sz_titems);
+     auto* d_progs = reinterpret_cast<BatchTileProg*>(reinterpret_cast<char*>(d_ones) + sz_ones);
+     auto* d_tops = reinterpret_cast<TileOp*>(reinterpret_cast<char*>(d_progs) + sz_progs);
      auto up = [](void* d, const void* h, size_t bytes) {
           if (bytes) check(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice), "hipMemcpy(descriptors)");
      };
@@ -544,6 +684,8 @@ void Batch::build()
      up(d_items, hp.items.data(), hp.items.size() * sizeof(BatchItem));
      up(d_titems, hp.titems.data(), hp.titems.size() * sizeof(BatchTileItem));
      up(d_ones, hp.ones.data(), hp.ones.size() * sizeof(uint64_t));
+     up(d_progs, hp.progs.data(), hp.progs.size() * sizeof(BatchTileProg));
+     up(d_tops, hp.tops.data(), hp.tops.size() * sizeof(TileOp));
 
      // Capture: the persistent launch runs on a forked stream beside the
      // layered chain; both join before the graph ends.
@@ -564,7 +706,11 @@ void Batch::build()
           check(launch_set_ones(d_arena_, prec_, d_ones, static_cast<int>(hp.ones.size()), s1_),
                 "launch_set_ones");
           for (const auto& ll: hp.launches) {
-               if (ll.tile)
+               if (ll.kind == kTileProgram)
+                    check(launch_batch_tprog(d_arena_, prec_, d_mats_, d_progs + ll.first,
+                                             static_cast<int>(ll.count), d_tops, ll.k, ll.blocks, s1_),
+                          "launch_batch_tprog");
+               else if (ll.kind == kTileForm)
                     check(launch_batch_tile(d_arena_, prec_, d_mats_, d_titems + ll.first,
                                             static_cast<int>(ll.count), ll.k, ll.blocks, s1_),
                           "launch_batch_tile");

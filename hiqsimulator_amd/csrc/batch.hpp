@@ -164,6 +164,19 @@ struct TileOp {
 struct TileHigh {
      uint8_t hpos[kTileHigh];  // ascending index bits >= 6
 };
+// One tile pass of one circuit inside a layered launch of tile programs.
+struct BatchTileProg {
+     uint64_t state_off;
+     uint64_t block_start;
+     uint32_t op_first;  // first op in the launch's op array
+     uint16_t nops;
+     uint8_t log_blocks;  // log2 of the item's tiles (n - kTileBits)
+     uint8_t hpos[kTileHigh];
+     uint8_t pad[7];
+};
+hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const BatchTileProg* items,
+                              int nitems, const TileOp* ops, int kmax, uint64_t total_blocks,
+                              hipStream_t s);
 // One tile pass over a state of 2^n amplitudes (n >= kTileBits): ops[0..nops)
 // (device array) with matrices in `mats` (device, state precision).
 hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& h, const TileOp* ops,
@@ -188,6 +201,8 @@ struct BatchStats {
      int wg_grid = 0;
      int kmax = 0;                  // widest fused pass
      uint64_t padded_launches = 0;  // layered launches whose 2-D grid has padding blocks
+     uint64_t tile_passes = 0;      // HBM passes run as tile programs
+     uint64_t tile_ops = 0;         // ops inside them
 };
 
 class Batch {
@@ -204,8 +219,12 @@ public:
      int64_t add_copies(int64_t c, int64_t count);
      // k_by_n[n] = fusion width for circuits of n qubits (0 = default 4);
      // circuits of more than wg_max_qubits qubits take the layered path, whose
-     // passes may use the wave-tile form from tile_min_qubits qubits on.
-     void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits);
+     // passes may use the wave-tile form from tile_min_qubits qubits on;
+     // from prog_min_qubits qubits on (at least 12) a layered circuit runs as
+     // tile programs instead (ops of at most k_by_n[n] qubits, several per
+     // HBM pass).
+     void set_plan(const std::vector<int>& k_by_n, int wg_max_qubits, int tile_min_qubits,
+                   int prog_min_qubits);
      // Plans and fuses every circuit and lays out the device descriptors on
      // the host, then checks every launch's blocks and addresses against the
      // arena, matrix and item bounds (throws on a violation).  Needs no device.
@@ -229,9 +248,10 @@ private:
           std::vector<BatchGate> gates;
           uint64_t state_off = 0;
      };
+     enum LaunchKind : int { kDirect = 0, kTileForm = 1, kTileProgram = 2 };
      struct LayerLaunch {
-          int k;
-          bool tile;
+          int k;  // fused width, or the widest op of a tile-program launch
+          int kind;
           size_t first, count;
           uint64_t blocks;
      };
@@ -246,7 +266,9 @@ private:
           int n_wg = 0, kmax_wg = 1;
           std::vector<BatchItem> items;
           std::vector<BatchTileItem> titems;
-          std::vector<int> item_n, titem_n;  // qubits of each item's circuit (checks)
+          std::vector<BatchTileProg> progs;  // tile-program items
+          std::vector<TileOp> tops;          // their ops
+          std::vector<int> item_n, titem_n, prog_n;  // qubits of each item's circuit (checks)
           std::vector<LayerLaunch> launches;
           std::vector<uint64_t> ones;
      };
@@ -265,6 +287,7 @@ private:
      std::vector<int> k_by_n_;
      int wg_max_qubits_ = 16;
      int tile_min_qubits_ = 0;
+     int prog_min_qubits_ = 31;
      bool built_ = false;
      bool device_ready_ = false;
      void* d_arena_ = nullptr;

@@ -31,6 +31,20 @@ __device__ __forceinline__ V basis_amp(bool one)
      return a;
 }
 
+// Item lookup of a block-mapped launch: the last item whose first block is
+// at or below b.
+template <typename Item>
+__device__ __forceinline__ int find_item(const Item* __restrict__ items, int nitems, uint64_t b)
+{
+     int lo = 0, hi = nitems - 1;
+     while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (items[mid].block_start <= b) lo = mid;
+          else hi = mid - 1;
+     }
+     return lo;
+}
+
 // One fused pass of one workgroup over a state of 2^n amplitudes at psi
 // (LDS or global memory); mat is the pass's matrix in LDS, pos ascending.
 // The rows are not unrolled: a row is read (uniform address, broadcast),
@@ -140,25 +154,23 @@ __global__ void __launch_bounds__(NT)
      }
 }
 
-// Tile program pass: workgroup b loads tile b (swizzled block order), runs
-// the op list on LDS and writes the tile back.  Chunk c (64 contiguous
-// amplitudes) of the tile sits at base + pdep(c, hpos).
+// One tile program on one tile: load the 2^kTileBits amplitudes of tile
+// `tix` (chunk c, 64 contiguous amplitudes, at base + pdep(c, hpos)) into
+// LDS, run ops[0..nops) with a workgroup barrier after each, write back.
 template <typename R, int KMAX, int NT>
-__global__ void __launch_bounds__(NT)
-    k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
-                const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz)
+__device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
+                                             const TileOp* __restrict__ ops, const int nops,
+                                             const uint8_t (&hpos)[kTileHigh], const uint64_t tix,
+                                             VT<R>* tile, VT<R>* smat)
 {
      using V = VT<R>;
      constexpr int TS = 1 << kTileBits;
      constexpr int NW = NT / 64;
      constexpr int PER = (TS / 64) / NW;  // chunks per wave
-     __shared__ V tile[TS];
-     __shared__ V smat[1 << (2 * KMAX)];
-     const uint64_t b = swizzle_block(block_linear(), swz);
-     uint64_t base = b << 6;
+     uint64_t base = tix << 6;
 #pragma unroll
      for (int j = 0; j < kTileHigh; ++j) {
-          const uint64_t lo = base & ((1ull << h.hpos[j]) - 1ull);
+          const uint64_t lo = base & ((1ull << hpos[j]) - 1ull);
           base = ((base ^ lo) << 1) | lo;
      }
      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -168,7 +180,7 @@ __global__ void __launch_bounds__(NT)
           const int c = w + i * NW;
           uint64_t off = 0;
 #pragma unroll
-          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << h.hpos[j];
+          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << hpos[j];
           v[i] = ld_nt(psi + base + off + lane);
      }
 #pragma unroll
@@ -195,9 +207,39 @@ __global__ void __launch_bounds__(NT)
           const int c = w + i * NW;
           uint64_t off = 0;
 #pragma unroll
-          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << h.hpos[j];
+          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << hpos[j];
           st_nt(psi + base + off + lane, tile[c * 64 + lane]);
      }
+}
+
+// Tile pass over one state: workgroup b runs the program on tile
+// swizzle(b).
+template <typename R, int KMAX, int NT>
+__global__ void __launch_bounds__(NT)
+    k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
+                const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz)
+{
+     __shared__ VT<R> tile[1 << kTileBits];
+     __shared__ VT<R> smat[1 << (2 * KMAX)];
+     tile_program<R, KMAX, NT>(psi, mats, ops, nops, h.hpos, swizzle_block(block_linear(), swz), tile, smat);
+}
+
+// Tile passes of many states: block b belongs to the item whose blocks
+// contain it; the item's tiles are visited in swizzled order.
+template <typename R, int KMAX, int NT>
+__global__ void __launch_bounds__(NT)
+    k_batch_tprog(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
+                  const BatchTileProg* __restrict__ items, const int nitems,
+                  const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
+{
+     __shared__ VT<R> tile[1 << kTileBits];
+     __shared__ VT<R> smat[1 << (2 * KMAX)];
+     const uint64_t b = block_linear();
+     if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
+     const BatchTileProg& it = items[find_item(items, nitems, b)];
+     const uint64_t tix = swizzle_block(b - it.block_start, swz < it.log_blocks ? swz : it.log_blocks);
+     tile_program<R, KMAX, NT>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile,
+                               smat);
 }
 
 template <int K, typename R>
@@ -240,20 +282,6 @@ __global__ void __launch_bounds__(kBatchLayerBlock)
 #pragma unroll
      for (int j = 0; j < D; ++j) v[j] = p[off[j]];
      group_matvec<K, V, R>(mats + it.mat_off, v, [&](int i, V o) { p[off[i]] = o; });
-}
-
-// Item lookup of a block-mapped launch: the last item whose first block is
-// at or below b.
-template <typename Item>
-__device__ __forceinline__ int find_item(const Item* __restrict__ items, int nitems, uint64_t b)
-{
-     int lo = 0, hi = nitems - 1;
-     while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (items[mid].block_start <= b) lo = mid;
-          else hi = mid - 1;
-     }
-     return lo;
 }
 
 template <int K, typename R>
@@ -482,6 +510,30 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 }  // namespace
+
+hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const BatchTileProg* items,
+                              int nitems, const TileOp* ops, int kmax, uint64_t total_blocks,
+                              hipStream_t s)
+{
+     if (nitems <= 0) return hipSuccess;
+     if (kmax < 1 || kmax > 5) return hipErrorInvalidValue;
+     const dim3 grid = grid_2d(total_blocks);
+     const int swz = get_block_swizzle();
+#define HQ_BTP(RR, KM, NT)                                                                       \
+     hipLaunchKernelGGL((k_batch_tprog<RR, KM, NT>), grid, dim3(NT), 0, s,                        \
+                        static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
+                        nitems, ops, total_blocks, swz)
+     if (prec == Prec::C128) {
+          if (kmax >= 5) HQ_BTP(double, 5, 512);
+          else HQ_BTP(double, 4, 512);
+     }
+     else {
+          if (kmax >= 5) HQ_BTP(float, 5, 512);
+          else HQ_BTP(float, 4, 512);
+     }
+#undef HQ_BTP
+     return hipGetLastError();
+}
 
 hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& h, const TileOp* ops,
                             int nops, int kmax, const void* mats, hipStream_t s)

@@ -596,12 +596,13 @@ int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int
 }
 
 int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits,
-                          int tile_min_qubits)
+                          int tile_min_qubits, int prog_min_qubits)
 {
      return guard([&] {
           need(b, "batch");
           if (len) need(k_by_n, "k_by_n");
-          b->b.set_plan(std::vector<int>(k_by_n, k_by_n + len), wg_max_qubits, tile_min_qubits);
+          b->b.set_plan(std::vector<int>(k_by_n, k_by_n + len), wg_max_qubits, tile_min_qubits,
+                        prog_min_qubits);
      });
 }
 
@@ -663,11 +664,12 @@ int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len)
           need(b, "batch");
           if (len) need(out, "out");
           const hq::BatchStats& s = b->b.stats();
-          const uint64_t v[11] = {s.circuits,       s.passes,      s.wg_circuits,
+          const uint64_t v[13] = {s.circuits,       s.passes,      s.wg_circuits,
                                   s.lds_circuits,   s.layer_launches, s.arena_bytes,
                                   s.matrix_bytes,   s.amp_passes,  static_cast<uint64_t>(s.wg_grid),
-                                  static_cast<uint64_t>(s.kmax), s.padded_launches};
-          for (size_t i = 0; i < len && i < 11; ++i) out[i] = v[i];
+                                  static_cast<uint64_t>(s.kmax), s.padded_launches, s.tile_passes,
+                                  s.tile_ops};
+          for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
      });
 }
 

@@ -496,7 +496,8 @@ public:
           ok(hiqsim_batch_add_copies(h_, index, count, &first));
           return first;
      }
-     void set_plan(const std::map<int, int>& k_by_n, int wg_max_qubits, int tile_min_qubits)
+     void set_plan(const std::map<int, int>& k_by_n, int wg_max_qubits, int tile_min_qubits,
+                   int prog_min_qubits)
      {
           std::vector<int32_t> v;
           for (auto& kv: k_by_n) {
@@ -504,7 +505,8 @@ public:
                if (v.size() <= static_cast<size_t>(kv.first)) v.resize(kv.first + 1, 0);
                v[kv.first] = kv.second;
           }
-          ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits, tile_min_qubits));
+          ok(hiqsim_batch_set_plan(h_, v.data(), v.size(), wg_max_qubits, tile_min_qubits,
+                                   prog_min_qubits));
      }
      void prepare()
      {
@@ -548,14 +550,15 @@ public:
      }
      py::dict stats() const
      {
-          uint64_t v[11] = {0};
-          ok(hiqsim_batch_stats(h_, v, 11));
-          static const char* names[11] = {"circuits",      "passes",         "wg_circuits",
+          uint64_t v[13] = {0};
+          ok(hiqsim_batch_stats(h_, v, 13));
+          static const char* names[13] = {"circuits",      "passes",         "wg_circuits",
                                           "lds_circuits",  "layer_launches", "arena_bytes",
                                           "matrix_bytes",  "amp_passes",     "wg_grid",
-                                          "kmax",          "padded_launches"};
+                                          "kmax",          "padded_launches", "tile_passes",
+                                          "tile_ops"};
           py::dict d;
-          for (int i = 0; i < 11; ++i) d[names[i]] = v[i];
+          for (int i = 0; i < 13; ++i) d[names[i]] = v[i];
           return d;
      }
 
@@ -781,7 +784,7 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("add_circuit", &PyBatch::add_circuit, py::arg("n_qubits"), py::arg("gates"))
          .def("add_copies", &PyBatch::add_copies, py::arg("index"), py::arg("count"))
          .def("set_plan", &PyBatch::set_plan, py::arg("k_by_n"), py::arg("wg_max_qubits") = 16,
-              py::arg("tile_min_qubits") = 0)
+              py::arg("tile_min_qubits") = 0, py::arg("prog_min_qubits") = 31)
          .def("prepare", &PyBatch::prepare)
          .def("build", &PyBatch::build)
          .def("run", &PyBatch::run)

@@ -237,9 +237,12 @@ int hiqsim_batch_add_copies(hiqsim_batch_t* b, int64_t index, int64_t count, int
  * circuits of more than wg_max_qubits (0..24) qubits are advanced by
  * layered many-workgroup launches, the others each by one workgroup of a
  * persistent launch.  Layered passes of circuits of at least
- * tile_min_qubits qubits may take the wave-tile form (31 = never). */
+ * tile_min_qubits qubits may take the wave-tile form (31 = never); layered
+ * circuits of at least prog_min_qubits (and 12) qubits run as tile
+ * programs, several ops of at most k_by_n[n] qubits per HBM pass (31 =
+ * never). */
 int hiqsim_batch_set_plan(hiqsim_batch_t* b, const int32_t* k_by_n, size_t len, int wg_max_qubits,
-                          int tile_min_qubits);
+                          int tile_min_qubits, int prog_min_qubits);
 /* Plans and fuses every circuit and checks every launch's blocks and
  * addresses against its buffers on the host; needs no device (stats are
  * filled as for a build on a 256-CU device). */
@@ -252,10 +255,11 @@ int hiqsim_batch_num_circuits(hiqsim_batch_t* b, int64_t* n);
 int hiqsim_batch_num_qubits(hiqsim_batch_t* b, int64_t index, int* n_qubits);
 /* Final state of circuit `index`: 2^n interleaved complex128 values. */
 int hiqsim_batch_get_state(hiqsim_batch_t* b, int64_t index, double* out);
-/* out[0..10] = circuits, fused passes, workgroup-owned circuits, LDS-resident
+/* out[0..12] = circuits, HBM passes, workgroup-owned circuits, LDS-resident
  * circuits, layered launches, arena bytes, matrix bytes, amplitude-passes
- * (sum over passes of 2^n), persistent workgroups, widest fusion width,
- * layered launches whose 2-D grid carries padding blocks. */
+ * (sum over HBM passes of 2^n), persistent workgroups, widest fusion width,
+ * layered launches whose 2-D grid carries padding blocks, tile-program
+ * passes, ops inside them. */
 int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len);
 
 /* ---- request loop (continuous batching) ---------------------------------

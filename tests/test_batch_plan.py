@@ -11,11 +11,11 @@ from hiqsimulator_amd.batching import batch_workload
 from hiqsimulator_amd.scheduling import plan_clusters
 
 
-def make(circuits, precision, k_by_n, wg_max):
+def make(circuits, precision, k_by_n, wg_max, prog_min=31):
     b = hq.Batch(precision)
     for n, g in circuits:
         b.add_circuit(n, g)
-    b.set_plan(k_by_n, wg_max)
+    b.set_plan(k_by_n, wg_max, 0, prog_min)
     return b
 
 
@@ -37,6 +37,21 @@ def test_prepare_checks_a_mixed_workload(precision, wg_max):
     assert st["lds_circuits"] == sum(1 for n, _ in circuits if n <= min(wg_max, lds_n))
     if wg_max < 21:
         assert st["layer_launches"] > 0
+
+
+@pytest.mark.parametrize("prog_min", [12, 16])
+def test_prepare_tile_programs(prog_min):
+    circuits = batch_workload(40, 6, n_range=(10, 20), depth_range=(3, 16))
+    b = make(circuits, "c128", {}, 11, prog_min)
+    b.prepare()
+    st = b.stats()
+    progs = [(n, g) for n, g in circuits if n > 11 and n >= prog_min]
+    assert (st["tile_passes"] > 0) == bool(progs)
+    expect = 0
+    for n, g in progs:
+        po, _, _ = hq._cppsim_amd.plan_tile_passes([x[1] for x in g], [x[2] for x in g], n, 4)
+        expect += max(po) + 1 if po else 0
+    assert st["tile_passes"] == expect
 
 
 def test_prepare_large_launches():

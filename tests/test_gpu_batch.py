@@ -38,21 +38,24 @@ def oracle_state(oracle_mod, n, gates):
 
 
 PLANS = {
-    # name: (k_by_n, wg_max_qubits, tile_min_qubits)
-    "k4_split14": ({}, 14, 0),
-    "kvar_all_wg": ({n: 2 + n % 4 for n in range(1, 18)}, 24, 0),
-    "k5_all_layered": ({n: 5 for n in range(1, 18)}, 0, 0),
-    "k2_split16": ({n: 2 for n in range(1, 18)}, 16, 0),
-    "k3_layered_direct": ({n: 3 for n in range(1, 18)}, 10, 31),
+    # name: (k_by_n, wg_max_qubits, tile_min_qubits, prog_min_qubits)
+    "k4_split14": ({}, 14, 0, 31),
+    "kvar_all_wg": ({n: 2 + n % 4 for n in range(1, 18)}, 24, 0, 31),
+    "k5_all_layered": ({n: 5 for n in range(1, 18)}, 0, 0, 31),
+    "k2_split16": ({n: 2 for n in range(1, 18)}, 16, 0, 31),
+    "k3_layered_direct": ({n: 3 for n in range(1, 18)}, 10, 31, 31),
+    "programs_from_12": ({n: 2 + n % 4 for n in range(1, 18)}, 8, 0, 12),
+    "programs_from_15": ({}, 11, 0, 15),
 }
 
 
 @pytest.mark.parametrize("plan", sorted(PLANS))
 def test_batch_c128_matches_oracle(engine_mod, oracle_mod, plan):
-    k_by_n, wg_max, tile_min = PLANS[plan]
+    k_by_n, wg_max, tile_min, prog_min = PLANS[plan]
     circuits = mixed_circuits(5)
     print("batch c128 plan", plan, flush=True)
-    b = make_batch(circuits, "c128", k_by_n, wg_max, tile_min_qubits=tile_min)
+    b = make_batch(circuits, "c128", k_by_n, wg_max, tile_min_qubits=tile_min,
+                   prog_min_qubits=prog_min)
     st = b.stats()
     assert st["circuits"] == len(circuits)
     b.run()
@@ -76,10 +79,11 @@ def test_batch_classes_are_all_exercised(engine_mod):
     assert st["layer_launches"] > 0
 
 
-def test_batch_c64_fidelity(engine_mod, oracle_mod):
+@pytest.mark.parametrize("prog_min", [31, 12])
+def test_batch_c64_fidelity(engine_mod, oracle_mod, prog_min):
     circuits = mixed_circuits(9)
-    print("batch c64", flush=True)
-    b = make_batch(circuits, "c64", {}, 14)
+    print("batch c64 prog_min", prog_min, flush=True)
+    b = make_batch(circuits, "c64", {}, 11, prog_min_qubits=prog_min)
     b.run()
     for i, (n, g) in enumerate(circuits):
         ref = oracle_state(oracle_mod, n, g)
