@@ -230,7 +230,8 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
 def bench_serving(count, rates, seed, log_fn):
     """Poisson request streams, n log-uniform in 12..28, depth ~ U{20..100},
     1024 samples or 8 Pauli expectations each, complex64; the request loop
-    against one-circuit-at-a-time SimulatorMPI on the same stream."""
+    (tile programs from 12 qubits) against one-circuit-at-a-time
+    SimulatorMPI (apply_tiled) on the same stream."""
     from hiqsimulator_amd.serving import make_server, run_plain, run_server, serving_workload
     out = []
     for rate in rates:
@@ -240,7 +241,7 @@ def bench_serving(count, rates, seed, log_fn):
         st = server.stats()
         del server
         log_fn("serving rate %g: server %s" % (rate, srv))
-        plain, _ = run_plain(reqs, "c64")
+        plain, _ = run_plain(reqs, "c64", tiled=True)
         log_fn("serving rate %g: plain %s" % (rate, plain))
         out.append({"rate_per_s": rate, "requests": count,
                     "server": {k: round(v, 3) for k, v in srv.items()},

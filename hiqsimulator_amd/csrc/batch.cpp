@@ -33,11 +33,33 @@ struct DeviceScope {
      explicit DeviceScope(int dev) { check(hipSetDevice(dev), "hipSetDevice"); }
 };
 
-struct PlannedPass {
-     int k = 0;
-     uint8_t pos[5] = {0, 0, 0, 0, 0};
-     std::vector<cplx> m;  // 4^k entries, row-major
-};
+template <class F>
+void parallel_for(size_t n, F&& f)
+{
+     size_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+     nt = std::min(nt, n);
+     if (nt <= 1) {
+          for (size_t i = 0; i < n; ++i) f(i);
+          return;
+     }
+     std::vector<std::thread> th;
+     std::vector<std::exception_ptr> err(nt);
+     for (size_t t = 0; t < nt; ++t)
+          th.emplace_back([&, t] {
+               try {
+                    for (size_t i = t; i < n; i += nt) f(i);
+               } catch (...) {
+                    err[t] = std::current_exception();
+               }
+          });
+     for (auto& x: th) x.join();
+     for (auto& e: err)
+          if (e) std::rethrow_exception(e);
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
 
 // Clusters of at most k qubits (lookahead packer), each fused into one dense
 // matrix over its ascending qubits; controls enter the matrix as diag(I, U)
@@ -82,13 +104,6 @@ std::vector<PlannedPass> plan_circuit(const std::vector<BatchGate>& gates, int k
      }
      return out;
 }
-
-// One tile pass of a circuit: its planned bits and its fused ops, whose
-// positions are tile ranks (0..5 = index bits 0..5, 6 + j = hpos[j]).
-struct PlannedTile {
-     uint8_t hpos[kTileHigh];
-     std::vector<PlannedPass> ops;
-};
 
 std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width)
 {
@@ -138,33 +153,6 @@ std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates,
      return out;
 }
 
-template <class F>
-void parallel_for(size_t n, F&& f)
-{
-     size_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-     nt = std::min(nt, n);
-     if (nt <= 1) {
-          for (size_t i = 0; i < n; ++i) f(i);
-          return;
-     }
-     std::vector<std::thread> th;
-     std::vector<std::exception_ptr> err(nt);
-     for (size_t t = 0; t < nt; ++t)
-          th.emplace_back([&, t] {
-               try {
-                    for (size_t i = t; i < n; i += nt) f(i);
-               } catch (...) {
-                    err[t] = std::current_exception();
-               }
-          });
-     for (auto& x: th) x.join();
-     for (auto& e: err)
-          if (e) std::rethrow_exception(e);
-}
-
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
-}  // namespace
 
 Batch::Batch(Prec prec, int device) : prec_(prec), device_(device)
 {

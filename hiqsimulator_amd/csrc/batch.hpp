@@ -189,6 +189,25 @@ struct BatchGate {
      std::vector<int64_t> ctrls;
 };
 
+// One fused pass of a planned circuit: 2^k x 2^k matrix (FP64, row-major)
+// over ascending positions.
+struct PlannedPass {
+     int k = 0;
+     uint8_t pos[5] = {0, 0, 0, 0, 0};
+     std::vector<std::complex<double>> m;
+};
+// One tile pass of a planned circuit: its planned bits and its fused ops,
+// whose positions are tile ranks (0..5 = index bits 0..5, 6 + j = hpos[j]).
+struct PlannedTile {
+     uint8_t hpos[kTileHigh];
+     std::vector<PlannedPass> ops;
+};
+// Clusters of at most k qubits (lookahead packer), each fused into one dense
+// matrix over its ascending qubits, controls folded in as diag(I, U).
+std::vector<PlannedPass> plan_circuit(const std::vector<BatchGate>& gates, int k);
+// Tile passes (plan_tile_passes) with ops of at most op_width qubits fused.
+std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width);
+
 struct BatchStats {
      uint64_t circuits = 0;
      uint64_t passes = 0;           // fused passes over all circuits

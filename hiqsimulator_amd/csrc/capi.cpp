@@ -708,6 +708,14 @@ int hiqsim_server_set_widths(hiqsim_server_t* s, const int32_t* k_by_n, size_t l
      });
 }
 
+int hiqsim_server_set_prog_min_qubits(hiqsim_server_t* s, int q)
+{
+     return guard([&] {
+          need(s, "server");
+          s->s.set_prog_min_qubits(q);
+     });
+}
+
 int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
                          const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,
                          const size_t* ctrl_off, int kind, uint32_t shots, uint64_t seed,

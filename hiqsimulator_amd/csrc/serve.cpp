@@ -58,6 +58,8 @@ using TileList = ItemListT<BatchTileItem>;
 struct StepLayer {
      ItemList direct;
      TileList tile[6];
+     ItemListT<BatchTileProg> progs;
+     int prog_kmax = 1;
 };
 
 }  // namespace
@@ -155,6 +157,12 @@ Server::~Server()
      (void)hipStreamDestroy(stream_);
 }
 
+void Server::set_prog_min_qubits(int q)
+{
+     if (q < 0 || q > 31) throw std::runtime_error("Server: prog_min_qubits must be 0..31");
+     prog_min_qubits_ = q;
+}
+
 void Server::set_widths(const std::vector<int>& k_by_n)
 {
      for (int k: k_by_n)
@@ -210,43 +218,44 @@ int64_t Server::submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint
      r.t_submit = now_s();
      const size_t ab = amp_bytes(prec_);
      if (!gates.empty()) {
-          GateList gl;
-          for (const auto& g: gates) {
-               gl.gate.push_back(g.targets);
-               gl.ctrl.push_back(g.ctrls);
-               gl.diag.push_back(false);
-          }
-          const std::vector<int> cluster_of = plan_clusters(gl, fusion_width(n), kPlanLookahead);
-          const int nc = *std::max_element(cluster_of.begin(), cluster_of.end()) + 1;
-          std::vector<std::vector<int>> members(nc);
-          for (size_t i = 0; i < gates.size(); ++i) members[cluster_of[i]].push_back(static_cast<int>(i));
-          std::vector<std::vector<cplx>> mats;
-          for (const auto& mem: members) {
-               Fusion f;
-               for (int gi: mem) {
-                    const BatchGate& g = gates[gi];
-                    Mat m(size_t(1) << g.targets.size());
-                    m.a = g.m;
-                    std::vector<int64_t> ids = g.targets;
-                    add_controls(m, ids, g.ctrls);
-                    f.insert(std::move(m), false, std::move(ids), {});
+          std::vector<std::vector<cplx>*> mats;
+          std::vector<PlannedPass> fused;
+          std::vector<PlannedTile> tiles;
+          if (n >= std::max(prog_min_qubits_, kTileBits)) {
+               tiles = plan_circuit_tiles(gates, n, fusion_width(n));
+               for (auto& t: tiles) {
+                    TPass tp{};
+                    std::memcpy(tp.hpos, t.hpos, kTileHigh);
+                    tp.op_first = static_cast<uint32_t>(r.tops.size());
+                    tp.nops = static_cast<uint32_t>(t.ops.size());
+                    for (auto& op: t.ops) {
+                         TileOp to{};
+                         to.mat_off = r.mat_entries;
+                         to.k = static_cast<uint8_t>(op.k);
+                         std::memcpy(to.rpos, op.pos, 5);
+                         r.kmax_op = std::max(r.kmax_op, op.k);
+                         r.tops.push_back(to);
+                         r.mat_entries += op.m.size();
+                         mats.push_back(&op.m);
+                    }
+                    r.tpasses.push_back(tp);
                }
-               Mat M;
-               std::vector<int64_t> ids, ctrls;
-               bool diag = false;
-               f.perform(M, ids, ctrls, diag);
-               if (ids.size() > static_cast<size_t>(kMaxWidth) || !ctrls.empty())
-                    throw std::runtime_error("Server: a fused cluster spans more than 5 qubits");
-               Pass p{static_cast<int>(ids.size()), {0, 0, 0, 0, 0}};
-               for (size_t l = 0; l < ids.size(); ++l) p.pos[l] = static_cast<uint8_t>(ids[l]);
-               r.passes.push_back(p);
-               r.pass_mat.push_back(r.mat_entries);
-               r.mat_entries += M.a.size();
-               mats.push_back(std::move(M.a));
+          }
+          else {
+               fused = plan_circuit(gates, fusion_width(n));
+               for (auto& p: fused) {
+                    Pass q{p.k, {0, 0, 0, 0, 0}};
+                    std::memcpy(q.pos, p.pos, 5);
+                    r.passes.push_back(q);
+                    r.pass_mat.push_back(r.mat_entries);
+                    r.mat_entries += p.m.size();
+                    mats.push_back(&p.m);
+               }
           }
           r.mats.resize(r.mat_entries * ab);
           uint64_t o = 0;
-          for (const auto& m: mats) {
+          for (const auto* mp: mats) {
+               const auto& m = *mp;
                for (size_t i = 0; i < m.size(); ++i, ++o) {
                     if (prec_ == Prec::C128) {
                          double* d = reinterpret_cast<double*>(r.mats.data()) + 2 * o;
@@ -388,9 +397,31 @@ std::vector<int64_t> Server::step(bool block)
      std::vector<StepLayer> layers(pps_);
      int kmax = 1;
      std::vector<int64_t> finishing;
+     std::vector<TileOp> step_ops;  // ops of this step's tile programs (absolute matrix offsets)
      for (int64_t id: active_) {
           Req& r = reqs_.at(id);
-          for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
+          const size_t total = r.tpasses.empty() ? r.passes.size() : r.tpasses.size();
+          for (int l = 0; l < pps_ && r.next + l < r.tpasses.size(); ++l) {
+               const TPass& tp = r.tpasses[r.next + l];
+               if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ || r.n < kTileBits)
+                    throw std::runtime_error("Server: internal bounds check failed");
+               BatchTileProg it{};
+               it.state_off = r.state_off;
+               it.op_first = static_cast<uint32_t>(step_ops.size());
+               it.nops = static_cast<uint16_t>(tp.nops);
+               it.log_blocks = static_cast<uint8_t>(r.n - kTileBits);
+               std::memcpy(it.hpos, tp.hpos, kTileHigh);
+               for (uint32_t o = tp.op_first; o < tp.op_first + tp.nops; ++o) {
+                    TileOp to = r.tops[o];
+                    to.mat_off += r.mat_off;
+                    if (to.mat_off + (uint64_t(1) << (2 * to.k)) > mat_cap_entries_)
+                         throw std::runtime_error("Server: internal bounds check failed");
+                    step_ops.push_back(to);
+               }
+               layers[l].prog_kmax = std::max(layers[l].prog_kmax, r.kmax_op);
+               layers[l].progs.add(it, uint64_t(1) << it.log_blocks);
+          }
+          for (int l = 0; l < pps_ && r.tpasses.empty() && r.next + l < r.passes.size(); ++l) {
                const Pass& p = r.passes[r.next + l];
                const uint64_t moff = r.mat_off + r.pass_mat[r.next + l];
                const bool tile_pays = p.pos[0] < get_tile_high_target() &&
@@ -423,7 +454,7 @@ std::vector<int64_t> Server::step(bool block)
                kmax = std::max(kmax, p.k);
                layers[l].direct.add(it, ((uint64_t(1) << it.log_groups) + kBatchLayerBlock - 1) / kBatchLayerBlock);
           }
-          if (r.next + pps_ >= r.passes.size()) finishing.push_back(id);
+          if (r.next + pps_ >= total) finishing.push_back(id);
      }
      // Outputs of the requests that finish in this step.
      ItemList sums;
@@ -466,9 +497,11 @@ std::vector<int64_t> Server::step(bool block)
      // Layout in the slot's staging buffer.
      const size_t o_init = 0;
      size_t o = align_up(init.items.size() * sizeof(BatchItem), 256);
-     std::vector<size_t> o_layer(pps_);
+     std::vector<size_t> o_layer(pps_), o_prog(pps_);
      std::vector<std::array<size_t, 6>> o_tile(pps_);
      for (int l = 0; l < pps_; ++l) {
+          o_prog[l] = o;
+          o += align_up(layers[l].progs.items.size() * sizeof(BatchTileProg), 256);
           o_layer[l] = o;
           o += align_up(layers[l].direct.items.size() * sizeof(BatchItem), 256);
           for (int k = 1; k <= kMaxWidth; ++k) {
@@ -476,6 +509,8 @@ std::vector<int64_t> Server::step(bool block)
                o += align_up(layers[l].tile[k].items.size() * sizeof(BatchTileItem), 256);
           }
      }
+     const size_t o_ops = o;
+     o += align_up(step_ops.size() * sizeof(TileOp), 256);
      const size_t o_sums = o;
      o += align_up(sums.items.size() * sizeof(BatchItem), 256);
      const size_t o_jobs = o;
@@ -483,7 +518,10 @@ std::vector<int64_t> Server::step(bool block)
      const size_t desc_bytes = std::max<size_t>(o, 256);
      ensure_slot(slot, desc_bytes, std::max<size_t>(out_bytes, 256));
      std::memcpy(slot.h_desc + o_init, init.items.data(), init.items.size() * sizeof(BatchItem));
+     std::memcpy(slot.h_desc + o_ops, step_ops.data(), step_ops.size() * sizeof(TileOp));
      for (int l = 0; l < pps_; ++l) {
+          std::memcpy(slot.h_desc + o_prog[l], layers[l].progs.items.data(),
+                      layers[l].progs.items.size() * sizeof(BatchTileProg));
           std::memcpy(slot.h_desc + o_layer[l], layers[l].direct.items.data(),
                       layers[l].direct.items.size() * sizeof(BatchItem));
           for (int k = 1; k <= kMaxWidth; ++k)
@@ -511,6 +549,15 @@ std::vector<int64_t> Server::step(bool block)
           ++stats_.launches;
      }
      for (int l = 0; l < pps_; ++l) {
+          if (!layers[l].progs.items.empty()) {
+               check(launch_batch_tprog(d_states_, prec_, d_mats_,
+                                        reinterpret_cast<const BatchTileProg*>(slot.d_desc + o_prog[l]),
+                                        static_cast<int>(layers[l].progs.items.size()),
+                                        reinterpret_cast<const TileOp*>(slot.d_desc + o_ops),
+                                        layers[l].prog_kmax, layers[l].progs.blocks, stream_),
+                     "launch_batch_tprog");
+               ++stats_.launches;
+          }
           for (int k = 1; k <= kMaxWidth; ++k) {
                const TileList& tl = layers[l].tile[k];
                if (tl.items.empty()) continue;
@@ -563,7 +610,8 @@ std::vector<int64_t> Server::step(bool block)
      std::vector<int64_t> still;
      for (int64_t id: active_) {
           Req& r = reqs_.at(id);
-          r.next = std::min(r.passes.size(), r.next + static_cast<size_t>(pps_));
+          const size_t total = r.tpasses.empty() ? r.passes.size() : r.tpasses.size();
+          r.next = std::min(total, r.next + static_cast<size_t>(pps_));
           if (std::find(finishing.begin(), finishing.end(), id) != finishing.end()) {
                r.state = 2;
                state_pool_.release(r.state_off, uint64_t(1) << r.n);

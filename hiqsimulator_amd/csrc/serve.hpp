@@ -8,9 +8,11 @@
 //    amplitudes) and fused matrices fit in the device pools (first fit, a
 //    request that does not fit lets later, smaller ones go first);
 //  * launches one step at a time: every admitted request advances by up to
-//    `passes_per_step` fused passes, pass l of all of them in one launch
-//    (widths mixed), so requests admitted while others run share the
-//    launches with them;
+//    `passes_per_step` HBM passes, pass l of all of them in one launch per
+//    form, so requests admitted while others run share the launches with
+//    them.  A request of at least prog_min_qubits (and 12) qubits runs as
+//    tile programs (several fused ops per HBM pass, applied in LDS); smaller
+//    ones as fused passes;
 //  * when a request's last pass is in a step, its outputs are computed in
 //    the same step and copied back; its memory returns to the pools at
 //    once (the stream orders later use after the outputs);
@@ -63,6 +65,9 @@ public:
      Server& operator=(const Server&) = delete;
 
      void set_widths(const std::vector<int>& k_by_n);
+     // Requests of at least this many qubits (and 12) run as tile programs
+     // (31 = never); applies to requests submitted afterwards.
+     void set_prog_min_qubits(int q);
      int64_t submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint32_t shots,
                     uint64_t seed, std::vector<uint64_t> xmask, std::vector<uint64_t> zmask);
      // Collects finished steps (waiting for the oldest one if `block` and
@@ -83,6 +88,11 @@ private:
           int k;
           uint8_t pos[5];
      };
+     struct TPass {
+          uint8_t hpos[kTileHigh];
+          uint32_t op_first;  // into Req::tops
+          uint32_t nops;
+     };
      struct Req {
           int n = 0;
           ServeKind kind = ServeKind::Samples;
@@ -91,6 +101,9 @@ private:
           std::vector<uint64_t> xmask, zmask;
           std::vector<Pass> passes;
           std::vector<uint64_t> pass_mat;  // first matrix entry of each pass (relative)
+          std::vector<TPass> tpasses;      // tile programs (instead of passes)
+          std::vector<TileOp> tops;        // their ops, mat_off relative
+          int kmax_op = 1;
           std::vector<char> mats;  // fused matrices in the state precision
           uint64_t mat_entries = 0;
           size_t next = 0;
@@ -130,6 +143,7 @@ private:
      uint64_t state_cap_amps_ = 0;
      uint64_t mat_cap_entries_ = 0;
      std::vector<int> k_by_n_;
+     int prog_min_qubits_ = 12;
      std::map<int64_t, Req> reqs_;
      std::deque<int64_t> waiting_;
      std::vector<int64_t> active_;

@@ -116,10 +116,12 @@ def run_server(server, reqs, keep_results=False):
     return latency_summary(list(lat.values()), makespan, len(reqs)), results
 
 
-def run_plain(reqs, precision="c64", k=4, keep_results=False, device=0):
+def run_plain(reqs, precision="c64", k=4, keep_results=False, device=0, tiled=True):
     """The same stream through one SimulatorMPI per request, one request at a
-    time in arrival order (gates fed cluster by cluster with run(), then
-    ``sample`` or one expectation per Pauli string)."""
+    time in arrival order: the gates go through ``apply_tiled`` (tile
+    passes, from 12 qubits on) or, with ``tiled=False`` or below 12 qubits,
+    cluster by cluster through apply_controlled_gate and run(); then
+    ``sample`` or one expectation per Pauli string."""
     order = sorted(range(len(reqs)), key=lambda i: reqs[i]["arrival"])
     lat = []
     results = {}
@@ -133,11 +135,14 @@ def run_plain(reqs, precision="c64", k=4, keep_results=False, device=0):
         sim = SimulatorMPI(1, n, k, precision, device)
         ids = list(range(n))
         sim.allocate_qureg(ids, 0)
-        for cl in plan_clusters(r["gates"], ids, k, mode="lookahead"):
-            for g in cl:
-                m, t, c = r["gates"][g]
-                sim.apply_controlled_gate(m, t, c)
-            sim.run()
+        if tiled and n >= 12:
+            sim.apply_tiled(r["gates"], k)
+        else:
+            for cl in plan_clusters(r["gates"], ids, k, mode="lookahead"):
+                for g in cl:
+                    m, t, c = r["gates"][g]
+                    sim.apply_controlled_gate(m, t, c)
+                sim.run()
         if r["kind"] == "samples":
             out = sim.sample(r["shots"], r["seed"])
         else:
@@ -165,9 +170,12 @@ def pauli_term(x, z):
 
 
 def make_server(precision="c64", state_bytes=32 << 30, matrix_bytes=2 << 30, passes_per_step=8,
-                max_active=1024, k_by_n=None, device=0):
+                max_active=1024, k_by_n=None, device=0, prog_min_qubits=12):
+    """A request loop with its pools; requests of at least prog_min_qubits
+    qubits run as tile programs (31 = never)."""
     s = Server(precision, device, int(state_bytes), int(matrix_bytes), int(passes_per_step),
                int(max_active))
     if k_by_n:
         s.set_widths(dict(k_by_n))
+    s.set_prog_min_qubits(int(prog_min_qubits))
     return s

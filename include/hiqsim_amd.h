@@ -277,6 +277,9 @@ int hiqsim_server_create(int precision, int device, uint64_t state_bytes, uint64
 int hiqsim_server_destroy(hiqsim_server_t* s);
 /* k_by_n[n] = fusion width (1..5, 0 = default 4) for requests of n qubits. */
 int hiqsim_server_set_widths(hiqsim_server_t* s, const int32_t* k_by_n, size_t len);
+/* Requests of at least q (and 12) qubits run as tile programs, several fused
+ * ops per HBM pass (default 12; 31 = never); for later submissions. */
+int hiqsim_server_set_prog_min_qubits(hiqsim_server_t* s, int q);
 /* Gates as in hiqsim_batch_add_circuit; masks hold nterms entries (kind 1). */
 int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
                          const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,

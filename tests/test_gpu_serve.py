@@ -52,13 +52,13 @@ def oracle_state_sim(oracle_mod, r):
     return o
 
 
-@pytest.mark.parametrize("precision", ["c128", "c64"])
-def test_same_alone_and_in_mix(engine_mod, oracle_mod, precision):
+@pytest.mark.parametrize("precision,prog_min", [("c128", 31), ("c64", 31), ("c128", 12)])
+def test_same_alone_and_in_mix(engine_mod, oracle_mod, precision, prog_min):
     reqs = small_requests(3)
-    print("serve mix", precision, flush=True)
-    mix = serve_all(reqs, precision, passes_per_step=3)
+    print("serve mix", precision, "prog_min", prog_min, flush=True)
+    mix = serve_all(reqs, precision, passes_per_step=3, prog_min_qubits=prog_min)
     for i, r in enumerate(reqs):
-        alone = serve_all([r], precision, passes_per_step=5)[0]
+        alone = serve_all([r], precision, passes_per_step=5, prog_min_qubits=prog_min)[0]
         if r["kind"] == "samples":
             assert np.array_equal(mix[i], alone), i
         else:
@@ -71,7 +71,7 @@ def test_same_alone_and_in_mix(engine_mod, oracle_mod, precision):
 
 def test_samples_match_engine_sampler(engine_mod):
     reqs = [r for r in small_requests(4) if r["kind"] == "samples"]
-    got = serve_all(reqs, "c128")
+    got = serve_all(reqs, "c128", prog_min_qubits=31)
     for r, g in zip(reqs, got):
         n = r["n"]
         sim = engine_mod.SimulatorMPI(1, n, 4)
