@@ -65,51 +65,6 @@ struct StepLayer {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-RangeAlloc::RangeAlloc(uint64_t capacity)
-{
-     if (capacity) free_[0] = capacity;
-     free_total_ = capacity;
-}
-
-bool RangeAlloc::alloc(uint64_t size, uint64_t align, uint64_t* off)
-{
-     if (size == 0) size = 1;
-     for (auto it = free_.begin(); it != free_.end(); ++it) {
-          const uint64_t start = (it->first + align - 1) / align * align;
-          const uint64_t end = it->first + it->second;
-          if (start + size > end) continue;
-          const uint64_t f_off = it->first, f_end = end;
-          free_.erase(it);
-          if (start > f_off) free_[f_off] = start - f_off;
-          if (start + size < f_end) free_[start + size] = f_end - (start + size);
-          free_total_ -= size;
-          *off = start;
-          return true;
-     }
-     return false;
-}
-
-void RangeAlloc::release(uint64_t off, uint64_t size)
-{
-     if (size == 0) size = 1;
-     auto next = free_.lower_bound(off);
-     uint64_t start = off, end = off + size;
-     if (next != free_.begin()) {
-          auto prev = std::prev(next);
-          if (prev->first + prev->second == start) {
-               start = prev->first;
-               free_.erase(prev);
-          }
-     }
-     if (next != free_.end() && next->first == end) {
-          end = next->first + next->second;
-          free_.erase(next);
-     }
-     free_[start] = end - start;
-     free_total_ += size;
-}
-
-// ---------------------------------------------------------------------------
 Server::Server(Prec prec, int device, uint64_t state_bytes, uint64_t matrix_bytes,
                int passes_per_step, int max_active)
     : prec_(prec), device_(device), pps_(passes_per_step), max_active_(static_cast<size_t>(max_active))

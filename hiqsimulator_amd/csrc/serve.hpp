@@ -31,21 +31,9 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "range_alloc.hpp"
 
 namespace hq {
-
-// First-fit allocator over [0, capacity) with coalescing of freed ranges.
-class RangeAlloc {
-public:
-     explicit RangeAlloc(uint64_t capacity = 0);
-     bool alloc(uint64_t size, uint64_t align, uint64_t* off);
-     void release(uint64_t off, uint64_t size);
-     uint64_t free_total() const { return free_total_; }
-
-private:
-     std::map<uint64_t, uint64_t> free_;  // offset -> size
-     uint64_t free_total_ = 0;
-};
 
 enum class ServeKind : int { Samples = 0, Expectation = 1 };
 
