@@ -302,7 +302,7 @@ def main():
     tuning = {name: hq._cppsim_amd.get_tuning(name) for name in ("block_swizzle", "tile_high_target")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
-    gates, step, tiled_step = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
+    gates, step, _ = build_step(n, args.layers, k, args.seed, args.perms, args.planner)
     n_pass = sum(1 for s in step if s[0] == "pass")
     n_perm = sum(1 for s in step if s[0] == "perm")
     plan_s = time.perf_counter() - t0
@@ -341,31 +341,39 @@ def main():
     achieved = 32.0 * 2 ** n / (per_launch_ms * 1e-3) / 1e9
     kernel_share = {name: round(ms, 2) for name, (c, ms) in times.items() if c}
 
-    # The same circuit with tile passes (the cluster scheduler targeting the
-    # LDS tile): fewer HBM passes, each applying several fused ops.
+    dense_ms_step = sum(ms for name, (c, ms) in times.items() if name.startswith("dense")) / args.steps
+    del sim
+
+    # The same gates with tile passes (the cluster scheduler targeting the
+    # LDS tile): fewer HBM passes, each applying several fused ops.  Run on a
+    # fresh state without the permutations; compared with the fused passes'
+    # kernel time of the run above.
     tile = None
     if not args.no_tile:
+        ts = hq.SimulatorMPI(args.seed, n, k)
+        ts.allocate_qureg(list(range(n)), 0)
         for w in range(args.warmup):
-            run_step(sim, tiled_step, k)
-        sim.synchronize()
-        sim.set_timing(True)
+            ts.apply_tiled(gates, k)
+        ts.synchronize()
+        ts.set_timing(True)
         t0 = time.perf_counter()
         tp = 0
         for s in range(args.steps):
-            tp = run_step(sim, tiled_step, k)
-        sim.synchronize()
+            tp = ts.apply_tiled(gates, k)
+        ts.synchronize()
         el_t = time.perf_counter() - t0
-        tcnt, ttot = sim.kernel_time(hq.KERNEL_CLASSES.index("tile"))
-        sim.set_timing(False)
+        tcnt, ttot = ts.kernel_time(hq.KERNEL_CLASSES.index("tile"))
+        ts.set_timing(False)
         tile_ms = ttot / tcnt if tcnt else float("nan")
-        tile = {"hbm_passes_per_step": tp, "fused_passes_per_step": n_pass,
-                "ms_per_step": round(el_t / args.steps * 1e3, 2),
-                "speedup_vs_fused": round(ms_step / (el_t / args.steps * 1e3), 3),
+        tile = {"hbm_passes_per_circuit": tp, "fused_passes_per_circuit": n_pass,
+                "ms_per_circuit": round(el_t / args.steps * 1e3, 2),
+                "fused_pass_kernel_ms_per_circuit": round(dense_ms_step, 2),
+                "speedup_vs_fused_kernels": round(dense_ms_step / (el_t / args.steps * 1e3), 3),
                 "tile_pass_avg_ms": round(tile_ms, 4),
                 "tile_pass_hbm_gbs": round(32.0 * 2 ** n / (tile_ms * 1e-3) / 1e9, 1),
-                "op_width": k, "final_norm": sim.get_probability([], [])}
+                "op_width": k, "final_norm": ts.get_probability([], [])}
+        del ts
         log("tile passes: %s" % tile)
-    del sim
 
     sweep = None if args.no_sweep else kernel_sweep(hq, n)
     if sweep:
