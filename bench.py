@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--serve-rates", default="20,60")
     ap.add_argument("--block-swizzle", type=int, default=None,
                     help="low block-index bits reversed in the streaming passes (library default 8)")
+    ap.add_argument("--block-swizzle-wide", type=int, default=None,
+                    help="the same for dense passes of k >= 4 (library default 6)")
     ap.add_argument("--tile-high-target", type=int, default=None,
                     help="dense passes with every target at or above this bit use the direct form")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -297,9 +299,12 @@ def main():
     import hiqsimulator_amd as hq
     if args.block_swizzle is not None:
         hq._cppsim_amd.set_tuning("block_swizzle", args.block_swizzle)
+    if args.block_swizzle_wide is not None:
+        hq._cppsim_amd.set_tuning("block_swizzle_wide", args.block_swizzle_wide)
     if args.tile_high_target is not None:
         hq._cppsim_amd.set_tuning("tile_high_target", args.tile_high_target)
-    tuning = {name: hq._cppsim_amd.get_tuning(name) for name in ("block_swizzle", "tile_high_target")}
+    tuning = {name: hq._cppsim_amd.get_tuning(name)
+              for name in ("block_swizzle", "block_swizzle_wide", "tile_high_target")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
     gates, step, _ = build_step(n, args.layers, k, args.seed, args.perms, args.planner)

@@ -99,6 +99,60 @@ __device__ __forceinline__ void wg_pass(VT<R>* psi, uint32_t n, const VT<R>* mat
      }
 }
 
+// The same pass with the matrix read from global memory at a uniform
+// address (scalar loads into SGPRs, shared by the whole wave), two rows at a
+// time: no LDS traffic for the matrix, and no SGPR spills (a fully unrolled
+// row loop would hold the whole matrix).
+template <int K, typename R, int NT>
+__device__ __forceinline__ void wg_pass_gm(VT<R>* psi, uint32_t n, const VT<R>* __restrict__ mat,
+                                           const uint8_t (&pos)[5])
+{
+     using V = VT<R>;
+     constexpr int D = 1 << K;
+     const uint32_t ng = 1u << (n - K);
+     uint32_t off[D];
+#pragma unroll
+     for (int j = 0; j < D; ++j) {
+          uint32_t o = 0;
+#pragma unroll
+          for (int l = 0; l < K; ++l)
+               if ((j >> l) & 1) o |= 1u << pos[l];
+          off[j] = o;
+     }
+     for (uint32_t t = threadIdx.x; t < ng; t += NT) {
+          uint32_t b = t;
+#pragma unroll
+          for (int l = 0; l < K; ++l) {
+               const uint32_t lo = b & ((1u << pos[l]) - 1u);
+               b = ((b ^ lo) << 1) | lo;
+          }
+          V v[D];
+#pragma unroll
+          for (int j = 0; j < D; ++j) v[j] = psi[b + off[j]];
+#pragma unroll 2
+          for (int i = 0; i < D; ++i) {
+               const V* const row = mat + i * D;
+               R re = 0, im = 0;
+#pragma unroll
+               for (int j = 0; j < D; ++j) {
+                    const V m = row[j];
+                    re = fma(m.x, v[j].x, re);
+                    re = fma(-m.y, v[j].y, re);
+                    im = fma(m.x, v[j].y, im);
+                    im = fma(m.y, v[j].x, im);
+               }
+               uint32_t oi = 0;
+#pragma unroll
+               for (int l = 0; l < K; ++l)
+                    if ((i >> l) & 1) oi |= 1u << pos[l];
+               V o;
+               o.x = re;
+               o.y = im;
+               psi[b + oi] = o;
+          }
+     }
+}
+
 // |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
 // staged in LDS (smat) and read from there with uniform-address loads, which
 // keeps the scalar registers free for the group offsets.
@@ -156,12 +210,13 @@ __global__ void __launch_bounds__(NT)
 
 // One tile program on one tile: load the 2^kTileBits amplitudes of tile
 // `tix` (chunk c, 64 contiguous amplitudes, at base + pdep(c, hpos)) into
-// LDS, run ops[0..nops) with a workgroup barrier after each, write back.
+// LDS, run ops[0..nops) with a workgroup barrier before each, write back.
+// Op matrices are read with uniform global loads.
 template <typename R, int KMAX, int NT>
 __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
                                              const TileOp* __restrict__ ops, const int nops,
                                              const uint8_t (&hpos)[kTileHigh], const uint64_t tix,
-                                             VT<R>* tile, VT<R>* smat)
+                                             VT<R>* tile)
 {
      using V = VT<R>;
      constexpr int TS = 1 << kTileBits;
@@ -187,19 +242,17 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
      for (int i = 0; i < PER; ++i) tile[(w + i * NW) * 64 + lane] = v[i];
      for (int p = 0; p < nops; ++p) {
           const TileOp o = ops[p];
-          const uint32_t dd = 1u << (2 * o.k);
-          for (uint32_t i = threadIdx.x; i < dd; i += NT) smat[i] = mats[o.mat_off + i];
-          __syncthreads();
+          const V* const m = mats + o.mat_off;
+          __syncthreads();  // the previous op (or the load) is complete
           switch (o.k) {
-               case 1: wg_pass<1, R, NT>(tile, kTileBits, smat, o.rpos); break;
-               case 2: wg_pass<2, R, NT>(tile, kTileBits, smat, o.rpos); break;
-               case 3: wg_pass<3, R, NT>(tile, kTileBits, smat, o.rpos); break;
-               case 4: wg_pass<4, R, NT>(tile, kTileBits, smat, o.rpos); break;
+               case 1: wg_pass_gm<1, R, NT>(tile, kTileBits, m, o.rpos); break;
+               case 2: wg_pass_gm<2, R, NT>(tile, kTileBits, m, o.rpos); break;
+               case 3: wg_pass_gm<3, R, NT>(tile, kTileBits, m, o.rpos); break;
+               case 4: wg_pass_gm<4, R, NT>(tile, kTileBits, m, o.rpos); break;
                default:
-                    if constexpr (KMAX >= 5) wg_pass<5, R, NT>(tile, kTileBits, smat, o.rpos);
+                    if constexpr (KMAX >= 5) wg_pass_gm<5, R, NT>(tile, kTileBits, m, o.rpos);
                     break;
           }
-          __syncthreads();  // the op is complete before smat is refilled
      }
      __syncthreads();
 #pragma unroll
@@ -220,8 +273,7 @@ __global__ void __launch_bounds__(NT)
                 const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[1 << (2 * KMAX)];
-     tile_program<R, KMAX, NT>(psi, mats, ops, nops, h.hpos, swizzle_block(block_linear(), swz), tile, smat);
+     tile_program<R, KMAX, NT>(psi, mats, ops, nops, h.hpos, swizzle_block(block_linear(), swz), tile);
 }
 
 // Tile passes of many states: block b belongs to the item whose blocks
@@ -233,13 +285,11 @@ __global__ void __launch_bounds__(NT)
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[1 << (2 * KMAX)];
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
      const BatchTileProg& it = items[find_item(items, nitems, b)];
      const uint64_t tix = swizzle_block(b - it.block_start, swz < it.log_blocks ? swz : it.log_blocks);
-     tile_program<R, KMAX, NT>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile,
-                               smat);
+     tile_program<R, KMAX, NT>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile);
 }
 
 template <int K, typename R>
@@ -597,11 +647,11 @@ hipError_t launch_batch_layer(void* arena, Prec prec, const void* mats, const Ba
                hipLaunchKernelGGL((k_batch_layer<KK, double>), grid, dim3(kBatchLayerBlock), 0,   \
                                   s, static_cast<double2*>(arena),                                \
                                   static_cast<const double2*>(mats), items, nitems, total_blocks, \
-                                  get_block_swizzle());                                           \
+                                  get_block_swizzle_for(KK));                                     \
           else                                                                                    \
                hipLaunchKernelGGL((k_batch_layer<KK, float>), grid, dim3(kBatchLayerBlock), 0, s, \
                                   static_cast<float2*>(arena), static_cast<const float2*>(mats),  \
-                                  items, nitems, total_blocks, get_block_swizzle());              \
+                                  items, nitems, total_blocks, get_block_swizzle_for(KK));        \
      } while (0)
      switch (k) {
           case 1: HQ_LAYER(1); break;
@@ -690,13 +740,13 @@ hipError_t launch_batch_tile(void* arena, Prec prec, const void* mats, const Bat
                constexpr int W = (64 << KK) * 16 > 16384 ? 2 : 4;                                 \
                hipLaunchKernelGGL((k_batch_tile<KK, double, W>), grid, dim3(64 * W), 0, s,        \
                                   static_cast<double2*>(arena), static_cast<const double2*>(mats), \
-                                  items, nitems, total_blocks, get_block_swizzle());              \
+                                  items, nitems, total_blocks, get_block_swizzle_for(KK));        \
           }                                                                                       \
           else {                                                                                  \
                constexpr int W = (64 << KK) * 8 > 16384 ? 2 : 4;                                  \
                hipLaunchKernelGGL((k_batch_tile<KK, float, W>), grid, dim3(64 * W), 0, s,         \
                                   static_cast<float2*>(arena), static_cast<const float2*>(mats),   \
-                                  items, nitems, total_blocks, get_block_swizzle());              \
+                                  items, nitems, total_blocks, get_block_swizzle_for(KK));        \
           }                                                                                       \
      } while (0)
      switch (k) {

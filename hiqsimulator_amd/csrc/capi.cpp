@@ -542,6 +542,10 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                if (value < 0 || value > 16) throw std::runtime_error("block_swizzle must be 0..16");
                hq::set_block_swizzle(static_cast<int>(value));
           }
+          else if (n == "block_swizzle_wide") {
+               if (value < 0 || value > 16) throw std::runtime_error("block_swizzle_wide must be 0..16");
+               hq::set_block_swizzle_wide(static_cast<int>(value));
+          }
           else
                throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
@@ -555,6 +559,7 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           const std::string n(name);
           if (n == "tile_high_target") *value = hq::get_tile_high_target();
           else if (n == "block_swizzle") *value = hq::get_block_swizzle();
+          else if (n == "block_swizzle_wide") *value = hq::get_block_swizzle_wide();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

@@ -586,17 +586,24 @@ constexpr int tile_wpb()
      return (64 << K) * static_cast<int>(sizeof(typename C2<R>::T)) > 16384 ? 2 : 4;
 }
 
-// Target bits at or above this position make the direct form faster than
-// the tile form (their chunks lie >= 2^g_high_target amplitudes apart).
-int g_high_target = 20;
-// Low block-index bits reversed by swizzle_block in the dense passes.
+// Dense passes whose targets all sit at or above this bit take the direct
+// form.  With the swizzled block order the tile form is as fast or faster
+// at every placement measured on MI355X (30 qubits, k = 1..4), so the
+// default never picks the direct form for that reason.
+int g_high_target = 64;
+// Low block-index bits reversed by swizzle_block: for passes of k <= 3 (and
+// the other streaming kernels) and for passes of k >= 4.  Measured on a
+// 30-qubit k = 4 pass over 30 placements: 5421 GB/s average without
+// swizzle, 5821 with 8 bits, 5940 with 6 bits.
 int g_swizzle_bits = 8;
+int g_swizzle_bits_wide = 6;
 
-// Swizzle width for a grid of `blocks` blocks: the configured width when the
-// grid holds whole runs of 2^bits blocks, else none.
-int swizzle_bits_for(uint64_t blocks)
+// Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
+// configured width when the grid holds whole runs of 2^bits blocks, else
+// none.
+int swizzle_bits_for(uint64_t blocks, int k)
 {
-     const int bits = g_swizzle_bits;
+     const int bits = get_block_swizzle_for(k);
      if (bits <= 0 || bits > 16) return 0;
      return (blocks % (1ull << bits)) == 0 ? bits : 0;
 }
@@ -654,7 +661,7 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
           const uint64_t blocks = ngroups / (64ull * WPB);
           hipLaunchKernelGGL((k_dense_tile<K, R, WPB>), grid_2d(blocks), dim3(64 * WPB), 0, s,
                              static_cast<V*>(psi), static_cast<const V*>(mat), d, L,
-                             swizzle_bits_for(blocks));
+                             swizzle_bits_for(blocks, K));
           return hipGetLastError();
      }
      // Direct form; non-temporal when every lane access covers whole 128-byte
@@ -665,7 +672,7 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      const uint64_t dblocks = (ngroups + kBlock * GPT - 1) / (kBlock * GPT);
      const dim3 grid = grid_2d(dblocks);
      // whole blocks only (a partial last block must stay last)
-     const int swz = (ngroups % (kBlock * GPT)) == 0 ? swizzle_bits_for(dblocks) : 0;
+     const int swz = (ngroups % (kBlock * GPT)) == 0 ? swizzle_bits_for(dblocks, K) : 0;
      if (nt)
           hipLaunchKernelGGL((k_dense_direct<K, R, GPT, true>), grid, dim3(kBlock), 0, s,
                              static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups, swz);
@@ -751,6 +758,9 @@ void set_tile_high_target(int pos) { g_high_target = pos; }
 int get_tile_high_target() { return g_high_target; }
 void set_block_swizzle(int bits) { g_swizzle_bits = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
 int get_block_swizzle() { return g_swizzle_bits; }
+void set_block_swizzle_wide(int bits) { g_swizzle_bits_wide = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
+int get_block_swizzle_wide() { return g_swizzle_bits_wide; }
+int get_block_swizzle_for(int k) { return k >= 4 ? g_swizzle_bits_wide : g_swizzle_bits; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -42,10 +42,15 @@ hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat,
 // every target bit is at or above this position (default 20).
 void set_tile_high_target(int pos);
 int get_tile_high_target();
-// Low block-index bits reversed in the dense passes (0 = identity order,
-// default 8, at most 16).
+// Low block-index bits reversed in the streaming passes (0 = identity
+// order, at most 16): `block_swizzle` for passes of k <= 3 and the other
+// streaming kernels (default 8), `block_swizzle_wide` for dense passes of
+// k >= 4 (default 6).
 void set_block_swizzle(int bits);
 int get_block_swizzle();
+void set_block_swizzle_wide(int bits);
+int get_block_swizzle_wide();
+int get_block_swizzle_for(int k);
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

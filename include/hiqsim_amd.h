@@ -203,10 +203,12 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
                             int32_t* pass_of, int32_t* op_of, int32_t* hpos, size_t* npasses);
 
 /* Process-wide kernel tuning (no reference counterpart):
- *   "tile_high_target": dense passes whose targets all sit at or above this
- *                       bit take the direct form (default 20);
- *   "block_swizzle":    low block-index bits reversed in the streaming
- *                       passes (0 = identity order, default 8, at most 16).
+ *   "tile_high_target":   dense passes whose targets all sit at or above this
+ *                         bit take the direct form (default 64: never);
+ *   "block_swizzle":      low block-index bits reversed in the streaming
+ *                         passes of k <= 3 and the other streaming kernels
+ *                         (0 = identity order, default 8, at most 16);
+ *   "block_swizzle_wide": the same for dense passes of k >= 4 (default 6).
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);
 int hiqsim_get_tuning(const char* name, int64_t* value);

@@ -145,12 +145,14 @@ def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
     order and form of the work: results equal the oracle at every setting."""
     import hiqsimulator_amd
     mod = hiqsimulator_amd._cppsim_amd
-    old = (mod.get_tuning("block_swizzle"), mod.get_tuning("tile_high_target"))
+    old = (mod.get_tuning("block_swizzle"), mod.get_tuning("tile_high_target"),
+           mod.get_tuning("block_swizzle_wide"))
     n = 22
     rng = np.random.default_rng(7 + swizzle)
     st = rand_state(rng, n)
     try:
         mod.set_tuning("block_swizzle", swizzle)
+        mod.set_tuning("block_swizzle_wide", swizzle)
         mod.set_tuning("tile_high_target", high)
         print("swizzle", swizzle, "high", high, flush=True)
         for pos in ([0, 1, 2, 3], [18, 19, 20, 21], [2, 9, 15, 21], [20], [0, 5, 10, 15, 20]):
@@ -165,3 +167,4 @@ def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
     finally:
         mod.set_tuning("block_swizzle", old[0])
         mod.set_tuning("tile_high_target", old[1])
+        mod.set_tuning("block_swizzle_wide", old[2])
