@@ -201,3 +201,30 @@ def workload_bytes(circuits, precision):
     """State bytes of a list of circuits."""
     ab = 16 if precision == "c128" else 8
     return sum(ab << n for n, _ in circuits)
+
+
+def main(argv=None):
+    """``python -m hiqsimulator_amd.batching``: builds the batch workload with
+    the stored tuning table and replays it; prints the plan and the replay
+    times (a short driver for profiling the batch kernels on their own)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--circuits", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
+    ap.add_argument("--replays", type=int, default=2)
+    a = ap.parse_args(argv)
+    circ = batch_workload(a.circuits, a.seed)
+    t = load_tuning(a.precision)
+    if t is None:
+        raise SystemExit("no tuning table for %s at %s" % (a.precision, tuning_path(a.precision)))
+    b = make_batch(circ, a.precision, t["k_by_n"], t["wg_max_qubits"],
+                   tile_min_qubits=t["tile_min_qubits"], prog_min_qubits=t["prog_min_qubits"])
+    print("plan", json.dumps(b.stats()), flush=True)
+    for i in range(a.replays):
+        print("replay %d: %.3f ms" % (i, b.run()), flush=True)
+
+
+if __name__ == "__main__":
+    main()
