@@ -288,6 +288,9 @@ def main():
                     help="the same for dense passes of k >= 4 (library default 6)")
     ap.add_argument("--tile-high-target", type=int, default=None,
                     help="dense passes with every target at or above this bit use the direct form")
+    ap.add_argument("--tile-mfma", type=int, default=None,
+                    help="ops of at least 3 qubits in tile programs run on the matrix cores (1) "
+                         "or on the vector units (0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()
@@ -303,8 +306,11 @@ def main():
         hq._cppsim_amd.set_tuning("block_swizzle_wide", args.block_swizzle_wide)
     if args.tile_high_target is not None:
         hq._cppsim_amd.set_tuning("tile_high_target", args.tile_high_target)
+    if args.tile_mfma is not None:
+        hq._cppsim_amd.set_tuning("tile_mfma", args.tile_mfma)
     tuning = {name: hq._cppsim_amd.get_tuning(name)
-              for name in ("block_swizzle", "block_swizzle_wide", "tile_high_target")}
+              for name in ("block_swizzle", "block_swizzle_wide", "tile_high_target",
+                           "tile_mfma")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()
     gates, step, _ = build_step(n, args.layers, k, args.seed, args.perms, args.planner)

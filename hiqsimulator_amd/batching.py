@@ -214,7 +214,13 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--replays", type=int, default=2)
+    ap.add_argument("--tile-mfma", type=int, default=None,
+                    help="ops of at least 3 qubits in tile programs run on the matrix cores (1) "
+                         "or on the vector units (0)")
     a = ap.parse_args(argv)
+    from . import _cppsim_amd
+    if a.tile_mfma is not None:
+        _cppsim_amd.set_tuning("tile_mfma", a.tile_mfma)
     circ = batch_workload(a.circuits, a.seed)
     t = load_tuning(a.precision)
     if t is None:

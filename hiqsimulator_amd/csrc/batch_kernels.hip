@@ -153,6 +153,136 @@ __device__ __forceinline__ void wg_pass_gm(VT<R>* psi, uint32_t n, const VT<R>* 
      }
 }
 
+// The same pass on the matrix cores, for 3 <= K <= 5.  The op is a real
+// product Out = A * In with In the 2D x (ng) matrix of the groups'
+// amplitudes (real and imaginary parts as separate rows) and A the 2D x 2D
+// real form of the complex matrix M:
+//   out.re = sum_j  Re M_oj x_j.re - Im M_oj x_j.im
+//   out.im = sum_j  Im M_oj x_j.re + Re M_oj x_j.im.
+// It runs as 16x16x4 MFMA tiles: 16 real output rows x 16 groups, 4 real
+// input rows per instruction.  The order of the real rows inside a tile is
+// chosen so that every lane moves whole complex amplitudes:
+//   * input: instruction s, lane (q = lane/16, c = lane%16) supplies part
+//     s%2 of amplitude 4*(s/2) + q of group c, so one LDS load of a complex
+//     amplitude feeds two instructions;
+//   * output: lane (q, c) receives four rows of its column (rows 4q..4q+3
+//     for the f32 instruction, rows q, q+4, q+8, q+12 for the f64 one, as
+//     measured on gfx950); the real rows are numbered so that these four are
+//     the real and imaginary parts of amplitudes 8*rb + 2q and 8*rb + 2q + 1
+//     of group c (rb = the 16-row block).
+// Every wave owns whole 16-group column blocks and reads all of a block's
+// amplitudes before its first write, so the pass stays in place without a
+// barrier of its own.
+template <typename R>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+     typedef double acc_t __attribute__((ext_vector_type(4)));
+     static __device__ __forceinline__ acc_t step(double a, double b, acc_t c)
+     {
+          return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+     }
+     // real row r (0..15) of a block: amplitude offset in the block, imaginary part?
+     static __device__ __forceinline__ int row_amp(int r) { return 2 * (r & 3) + (r >> 3); }
+     static __device__ __forceinline__ bool row_im(int r) { return (r >> 2) & 1; }
+};
+template <>
+struct Mfma16<float> {
+     typedef float acc_t __attribute__((ext_vector_type(4)));
+     static __device__ __forceinline__ acc_t step(float a, float b, acc_t c)
+     {
+          return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+     }
+     static __device__ __forceinline__ int row_amp(int r) { return r >> 1; }
+     static __device__ __forceinline__ bool row_im(int r) { return r & 1; }
+};
+
+template <int K>
+__device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
+{
+     uint32_t o = 0;
+#pragma unroll
+     for (int l = 0; l < K; ++l)
+          if ((j >> l) & 1) o |= 1u << pos[l];
+     return o;
+}
+
+template <int K, typename R, int NT>
+__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, uint32_t n, const VT<R>* __restrict__ mat,
+                                             const uint8_t (&pos)[5])
+{
+     static_assert(K >= 3 && K <= 5, "MFMA pass needs 3 <= K <= 5");
+     using V = VT<R>;
+     using M = Mfma16<R>;
+     using acc_t = typename M::acc_t;
+     constexpr int D = 1 << K;
+     constexpr int RB = D / 8;  // 16-row blocks of the real output
+     constexpr int T = D / 4;   // complex input amplitudes per lane
+     constexpr int NW = NT / 64;
+     constexpr bool kHoldA = RB * T <= 8;
+     const uint32_t ncb = (1u << (n - K)) >> 4;
+     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+     const int q = lane >> 4, c = lane & 15;
+     // A fragments: lane (q, c) holds row c of block rb (output amplitude
+     // 8*rb + row_amp(c), part row_im(c)) at input amplitudes 4t + q.
+     auto load_a = [&](int rb, R (&a)[2 * T]) {
+          const int o = rb * 8 + M::row_amp(c);
+          const bool im_row = M::row_im(c);
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+               const V m = mat[o * D + 4 * t + q];
+               a[2 * t] = im_row ? m.y : m.x;       // times x.re
+               a[2 * t + 1] = im_row ? m.x : -m.y;  // times x.im
+          }
+     };
+     R ahold[kHoldA ? RB : 1][2 * T];
+     if constexpr (kHoldA) {
+#pragma unroll
+          for (int rb = 0; rb < RB; ++rb) load_a(rb, ahold[rb]);
+     }
+     uint32_t in_off[T];
+#pragma unroll
+     for (int t = 0; t < T; ++t) in_off[t] = amp_offset<K>(4 * t + q, pos);
+     for (uint32_t cb = w; cb < ncb; cb += NW) {
+          uint32_t b = cb * 16 + c;
+#pragma unroll
+          for (int l = 0; l < K; ++l) {
+               const uint32_t lo = b & ((1u << pos[l]) - 1u);
+               b = ((b ^ lo) << 1) | lo;
+          }
+          V x[T];
+#pragma unroll
+          for (int t = 0; t < T; ++t) x[t] = psi[b + in_off[t]];
+          auto block = [&](const R (&a)[2 * T], int rb) {
+               acc_t acc = {0, 0, 0, 0};
+#pragma unroll
+               for (int t = 0; t < T; ++t) {
+                    acc = M::step(a[2 * t], x[t].x, acc);
+                    acc = M::step(a[2 * t + 1], x[t].y, acc);
+               }
+               const int o = rb * 8 + 2 * q;
+               V y0, y1;
+               y0.x = acc[0];
+               y0.y = acc[1];
+               y1.x = acc[2];
+               y1.y = acc[3];
+               psi[b + amp_offset<K>(o, pos)] = y0;
+               psi[b + amp_offset<K>(o + 1, pos)] = y1;
+          };
+#pragma unroll
+          for (int rb = 0; rb < RB; ++rb) {
+               if constexpr (kHoldA) {
+                    block(ahold[rb], rb);
+               }
+               else {
+                    R a[2 * T];
+                    load_a(rb, a);
+                    block(a, rb);
+               }
+          }
+     }
+}
+
 // |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
 // staged in LDS (smat) and read from there with uniform-address loads, which
 // keeps the scalar registers free for the group offsets.
@@ -208,88 +338,257 @@ __global__ void __launch_bounds__(NT)
      }
 }
 
-// One tile program on one tile: load the 2^kTileBits amplitudes of tile
-// `tix` (chunk c, 64 contiguous amplitudes, at base + pdep(c, hpos)) into
-// LDS, run ops[0..nops) with a workgroup barrier before each, write back.
-// Op matrices are read with uniform global loads.
-template <typename R, int KMAX, int NT>
-__device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
-                                             const TileOp* __restrict__ ops, const int nops,
-                                             const uint8_t (&hpos)[kTileHigh], const uint64_t tix,
-                                             VT<R>* tile)
+// Tile programs.  A tile is the 2^kTileBits amplitudes whose index bits
+// outside {0..5} u hpos are fixed: 64 chunks of 64 contiguous amplitudes,
+// chunk c at base + pdep(c, hpos).  One workgroup loads its tile into LDS,
+// runs the program's ops there (a workgroup barrier before each) and writes
+// the tile back; with two or three workgroups per CU, one workgroup's ops
+// overlap the others' HBM traffic (measured on a 30-qubit state: up to two
+// 4-qubit or four 3-qubit ops per tile cost no time over the bare copy).
+//
+// Ops of at least MK qubits run on the matrix cores (wg_pass_mfma), the
+// others on the vector units with the matrix read by uniform scalar loads
+// (MK = 6: all on the vector units).  The matrix-core ops read their
+// matrices from an LDS copy (smat) loaded together with the tile, so the
+// op phase issues no global loads.
+// A descriptor read as 32-bit words at a wave-uniform address: the words
+// come through scalar loads (a byte field read on its own would be a vector
+// load, whose wait would also wait for the tile loads in flight).
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* __restrict__ p)
 {
+     static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4");
+     constexpr int N = sizeof(T) / 4;
+     const uint32_t* const q = reinterpret_cast<const uint32_t*>(p);
+     uint32_t w[N];
+#pragma unroll
+     for (int i = 0; i < N; ++i) w[i] = q[i];
+     T r;
+     __builtin_memcpy(&r, w, sizeof(T));
+     return r;
+}
+
+template <typename R, int NT>
+struct TileIo {
      using V = VT<R>;
-     constexpr int TS = 1 << kTileBits;
-     constexpr int NW = NT / 64;
-     constexpr int PER = (TS / 64) / NW;  // chunks per wave
-     uint64_t base = tix << 6;
+     static constexpr int NW = NT / 64;
+     static constexpr int PER = (64 / NW);  // chunks per wave
+
+     static __device__ __forceinline__ int wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+     static __device__ __forceinline__ uint64_t base_of(uint64_t tix, const uint8_t (&hpos)[kTileHigh])
+     {
+          uint64_t base = tix << 6;
 #pragma unroll
-     for (int j = 0; j < kTileHigh; ++j) {
-          const uint64_t lo = base & ((1ull << hpos[j]) - 1ull);
-          base = ((base ^ lo) << 1) | lo;
+          for (int j = 0; j < kTileHigh; ++j) {
+               const uint64_t lo = base & ((1ull << hpos[j]) - 1ull);
+               base = ((base ^ lo) << 1) | lo;
+          }
+          return base;
      }
-     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-     V v[PER];
-#pragma unroll
-     for (int i = 0; i < PER; ++i) {
-          const int c = w + i * NW;
+     static __device__ __forceinline__ uint64_t chunk_off(int c, const uint8_t (&hpos)[kTileHigh])
+     {
           uint64_t off = 0;
 #pragma unroll
           for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << hpos[j];
-          v[i] = ld_nt(psi + base + off + lane);
+          return off;
      }
+     static __device__ __forceinline__ void load(const V* __restrict__ psi, uint64_t tix,
+                                                 const uint8_t (&hpos)[kTileHigh], V (&r)[PER])
+     {
+          const int lane = threadIdx.x & 63, w = wave();
+          const V* const p = psi + base_of(tix, hpos) + lane;
 #pragma unroll
-     for (int i = 0; i < PER; ++i) tile[(w + i * NW) * 64 + lane] = v[i];
+          for (int i = 0; i < PER; ++i) r[i] = ld_nt(p + chunk_off(w + i * NW, hpos));
+     }
+     static __device__ __forceinline__ void store(V* __restrict__ psi, uint64_t tix,
+                                                  const uint8_t (&hpos)[kTileHigh], const V (&r)[PER])
+     {
+          const int lane = threadIdx.x & 63, w = wave();
+          V* const p = psi + base_of(tix, hpos) + lane;
+#pragma unroll
+          for (int i = 0; i < PER; ++i) st_nt(p + chunk_off(w + i * NW, hpos), r[i]);
+     }
+     static __device__ __forceinline__ void to_lds(V* tile, const V (&r)[PER])
+     {
+          const int lane = threadIdx.x & 63, w = wave();
+#pragma unroll
+          for (int i = 0; i < PER; ++i) tile[(w + i * NW) * 64 + lane] = r[i];
+     }
+     static __device__ __forceinline__ void from_lds(const V* tile, V (&r)[PER])
+     {
+          const int lane = threadIdx.x & 63, w = wave();
+#pragma unroll
+          for (int i = 0; i < PER; ++i) r[i] = tile[(w + i * NW) * 64 + lane];
+     }
+};
+
+// LDS copy of the matrices of one tile program (matrix-core variants): the
+// ops' matrices back to back in op order, as far as kStage elements go; ops
+// beyond that read global memory.
+template <typename R>
+constexpr int tile_stage_elems()
+{
+     return sizeof(R) == 8 ? 1024 : 2048;  // 16 KiB
+}
+
+template <typename R, int NT>
+struct MatStage {
+     using V = VT<R>;
+     static constexpr int ME = tile_stage_elems<R>();
+     static constexpr int PER = ME / NT;
+
+     // Loads the matrices of ops[0..nops) that fit into r; returns the
+     // number of staged elements.  Addresses first (scalar walk over the
+     // ops, per-lane selects), then one unconditional load per register, so
+     // no load waits for another.
+     static __device__ __forceinline__ int load(V (&r)[PER], const V* __restrict__ mats,
+                                                const TileOp* __restrict__ ops, int nops)
+     {
+          uint64_t addr[PER];
+#pragma unroll
+          for (int i = 0; i < PER; ++i) addr[i] = 0;
+          int start = 0;
+          for (int p = 0; p < nops; ++p) {
+               const TileOp o = load_uniform(ops + p);
+               const int sz = 1 << (2 * o.k);
+               if (start + sz > ME) break;
+#pragma unroll
+               for (int i = 0; i < PER; ++i) {
+                    const int e = static_cast<int>(threadIdx.x) + i * NT;
+                    if (e >= start && e < start + sz) addr[i] = o.mat_off + static_cast<uint64_t>(e - start);
+               }
+               start += sz;
+          }
+#pragma unroll
+          for (int i = 0; i < PER; ++i) r[i] = mats[addr[i]];
+          return start;
+     }
+     static __device__ __forceinline__ void to_lds(V* smat, const V (&r)[PER], int n)
+     {
+#pragma unroll
+          for (int i = 0; i < PER; ++i) {
+               const int e = static_cast<int>(threadIdx.x) + i * NT;
+               if (e < n) smat[e] = r[i];
+          }
+     }
+};
+
+// Waves per SIMD the tile kernels are built for: two 512-thread workgroups
+// per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
+// VGPRs), three for complex64 (48 KiB; at most 80 VGPRs); complex128
+// programs with 5-qubit ops are left unconstrained (their 5-qubit pass
+// needs more registers).
+template <typename R, int KMAX>
+constexpr int tile_waves_per_eu()
+{
+     return sizeof(R) == 8 ? (KMAX >= 5 ? 1 : 4) : 6;
+}
+
+// ops[0..nops) on the tile in LDS, a workgroup barrier before each.  The
+// first `staged` elements of the ops' matrices are in smat (back to back in
+// op order).
+template <typename R, int KMAX, int NT, int MK>
+__device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const int staged,
+                                         const VT<R>* __restrict__ mats,
+                                         const TileOp* __restrict__ ops, const int nops)
+{
+     using V = VT<R>;
+     int soff = 0;
      for (int p = 0; p < nops; ++p) {
-          const TileOp o = ops[p];
+          const TileOp o = load_uniform(ops + p);
+          const int sz = 1 << (2 * o.k);
+          const bool in_lds = soff + sz <= staged;
+          __syncthreads();  // the previous op (or the tile load) is complete
+          if constexpr (MK <= 5) {
+               if (o.k >= MK) {
+                    if (in_lds) {
+                         const V* const m = smat + soff;
+                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, kTileBits, m, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, kTileBits, m, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, kTileBits, m, o.rpos);
+                    }
+                    else {
+                         const V* const m = mats + o.mat_off;
+                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, kTileBits, m, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, kTileBits, m, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, kTileBits, m, o.rpos);
+                    }
+                    soff += sz;
+                    continue;
+               }
+          }
+          soff += sz;
           const V* const m = mats + o.mat_off;
-          __syncthreads();  // the previous op (or the load) is complete
           switch (o.k) {
                case 1: wg_pass_gm<1, R, NT>(tile, kTileBits, m, o.rpos); break;
                case 2: wg_pass_gm<2, R, NT>(tile, kTileBits, m, o.rpos); break;
-               case 3: wg_pass_gm<3, R, NT>(tile, kTileBits, m, o.rpos); break;
-               case 4: wg_pass_gm<4, R, NT>(tile, kTileBits, m, o.rpos); break;
+               case 3:
+                    if constexpr (MK > 3) wg_pass_gm<3, R, NT>(tile, kTileBits, m, o.rpos);
+                    break;
+               case 4:
+                    if constexpr (MK > 4) wg_pass_gm<4, R, NT>(tile, kTileBits, m, o.rpos);
+                    break;
                default:
-                    if constexpr (KMAX >= 5) wg_pass_gm<5, R, NT>(tile, kTileBits, m, o.rpos);
+                    if constexpr (KMAX >= 5 && MK > 5) wg_pass_gm<5, R, NT>(tile, kTileBits, m, o.rpos);
                     break;
           }
      }
+}
+
+// One tile program on tile `tix` of the state at psi.
+template <typename R, int KMAX, int NT, int MK>
+__device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
+                                             const TileOp* __restrict__ ops, const int nops,
+                                             const uint8_t (&hpos)[kTileHigh], const uint64_t tix,
+                                             VT<R>* tile, VT<R>* smat)
+{
+     using V = VT<R>;
+     using IO = TileIo<R, NT>;
+     using MS = MatStage<R, NT>;
+     V v[IO::PER];
+     IO::load(psi, tix, hpos, v);
+     int staged = 0;
+     V mr[MK <= 5 ? MS::PER : 1];
+     if constexpr (MK <= 5) staged = MS::load(mr, mats, ops, nops);
+     IO::to_lds(tile, v);
+     if constexpr (MK <= 5) MS::to_lds(smat, mr, staged);
+     tile_ops<R, KMAX, NT, MK>(tile, smat, staged, mats, ops, nops);
      __syncthreads();
-#pragma unroll
-     for (int i = 0; i < PER; ++i) {
-          const int c = w + i * NW;
-          uint64_t off = 0;
-#pragma unroll
-          for (int j = 0; j < kTileHigh; ++j) off |= static_cast<uint64_t>((c >> j) & 1) << hpos[j];
-          st_nt(psi + base + off + lane, tile[c * 64 + lane]);
-     }
+     IO::from_lds(tile, v);
+     IO::store(psi, tix, hpos, v);
 }
 
 // Tile pass over one state: workgroup b runs the program on tile
 // swizzle(b).
-template <typename R, int KMAX, int NT>
-__global__ void __launch_bounds__(NT)
+template <typename R, int KMAX, int NT, int MK>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
     k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
-                const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz)
+                const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz,
+                const uint64_t nblocks)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     tile_program<R, KMAX, NT>(psi, mats, ops, nops, h.hpos, swizzle_block(block_linear(), swz), tile);
+     __shared__ VT<R> smat[MK <= 5 ? tile_stage_elems<R>() : 1];
+     const uint64_t b = block_linear();
+     if (b >= nblocks) return;  // padding of the 2-D grid (whole block)
+     tile_program<R, KMAX, NT, MK>(psi, mats, ops, nops, h.hpos, swizzle_block(b, swz), tile, smat);
 }
 
 // Tile passes of many states: block b belongs to the item whose blocks
 // contain it; the item's tiles are visited in swizzled order.
-template <typename R, int KMAX, int NT>
-__global__ void __launch_bounds__(NT)
+template <typename R, int KMAX, int NT, int MK>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
     k_batch_tprog(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
                   const BatchTileProg* __restrict__ items, const int nitems,
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
+     __shared__ VT<R> smat[MK <= 5 ? tile_stage_elems<R>() : 1];
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
-     const BatchTileProg& it = items[find_item(items, nitems, b)];
+     const BatchTileProg it = load_uniform(items + find_item(items, nitems, b));
      const uint64_t tix = swizzle_block(b - it.block_start, swz < it.log_blocks ? swz : it.log_blocks);
-     tile_program<R, KMAX, NT>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile);
+     tile_program<R, KMAX, NT, MK>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile,
+                                   smat);
 }
 
 template <int K, typename R>
@@ -565,22 +864,27 @@ hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const Ba
                               int nitems, const TileOp* ops, int kmax, uint64_t total_blocks,
                               hipStream_t s)
 {
-     if (nitems <= 0) return hipSuccess;
+     if (nitems <= 0 || total_blocks == 0) return hipSuccess;
      if (kmax < 1 || kmax > 5) return hipErrorInvalidValue;
      const dim3 grid = grid_2d(total_blocks);
      const int swz = get_block_swizzle();
-#define HQ_BTP(RR, KM, NT)                                                                       \
-     hipLaunchKernelGGL((k_batch_tprog<RR, KM, NT>), grid, dim3(NT), 0, s,                        \
+     const bool mfma = get_tile_mfma() != 0;
+#define HQ_BTP(RR, KM, MKV)                                                                       \
+     hipLaunchKernelGGL((k_batch_tprog<RR, KM, 512, MKV>), grid, dim3(512), 0, s,                 \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
                         nitems, ops, total_blocks, swz)
+#define HQ_BTP2(RR, KM)          \
+     if (mfma) HQ_BTP(RR, KM, 3); \
+     else HQ_BTP(RR, KM, 6)
      if (prec == Prec::C128) {
-          if (kmax >= 5) HQ_BTP(double, 5, 512);
-          else HQ_BTP(double, 4, 512);
+          if (kmax >= 5) HQ_BTP2(double, 5);
+          else HQ_BTP2(double, 4);
      }
      else {
-          if (kmax >= 5) HQ_BTP(float, 5, 512);
-          else HQ_BTP(float, 4, 512);
+          if (kmax >= 5) HQ_BTP2(float, 5);
+          else HQ_BTP2(float, 4);
      }
+#undef HQ_BTP2
 #undef HQ_BTP
      return hipGetLastError();
 }
@@ -597,17 +901,23 @@ hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& 
      const int bits = get_block_swizzle();
      const int swz = (bits > 0 && bits <= 16 && blocks % (1ull << bits) == 0) ? bits : 0;
      const dim3 grid = grid_2d(blocks);
-#define HQ_TP(RR, KM, NT)                                                                        \
-     hipLaunchKernelGGL((k_tile_pass<RR, KM, NT>), grid, dim3(NT), 0, s, static_cast<VT<RR>*>(psi), \
-                        static_cast<const VT<RR>*>(mats), ops, nops, h, swz)
+     const bool mfma = get_tile_mfma() != 0;
+#define HQ_TP(RR, KM, MKV)                                                                        \
+     hipLaunchKernelGGL((k_tile_pass<RR, KM, 512, MKV>), grid, dim3(512), 0, s,                   \
+                        static_cast<VT<RR>*>(psi), static_cast<const VT<RR>*>(mats), ops, nops, h, \
+                        swz, blocks)
+#define HQ_TP2(RR, KM)          \
+     if (mfma) HQ_TP(RR, KM, 3); \
+     else HQ_TP(RR, KM, 6)
      if (prec == Prec::C128) {
-          if (kmax >= 5) HQ_TP(double, 5, 512);
-          else HQ_TP(double, 4, 512);
+          if (kmax >= 5) HQ_TP2(double, 5);
+          else HQ_TP2(double, 4);
      }
      else {
-          if (kmax >= 5) HQ_TP(float, 5, 512);
-          else HQ_TP(float, 4, 512);
+          if (kmax >= 5) HQ_TP2(float, 5);
+          else HQ_TP2(float, 4);
      }
+#undef HQ_TP2
 #undef HQ_TP
      return hipGetLastError();
 }

@@ -546,6 +546,10 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                if (value < 0 || value > 16) throw std::runtime_error("block_swizzle_wide must be 0..16");
                hq::set_block_swizzle_wide(static_cast<int>(value));
           }
+          else if (n == "tile_mfma") {
+               if (value < 0 || value > 1) throw std::runtime_error("tile_mfma must be 0 or 1");
+               hq::set_tile_mfma(static_cast<int>(value));
+          }
           else
                throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
@@ -560,6 +564,7 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           if (n == "tile_high_target") *value = hq::get_tile_high_target();
           else if (n == "block_swizzle") *value = hq::get_block_swizzle();
           else if (n == "block_swizzle_wide") *value = hq::get_block_swizzle_wide();
+          else if (n == "tile_mfma") *value = hq::get_tile_mfma();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

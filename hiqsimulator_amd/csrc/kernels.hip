@@ -597,6 +597,7 @@ int g_high_target = 64;
 // swizzle, 5821 with 8 bits, 5940 with 6 bits.
 int g_swizzle_bits = 8;
 int g_swizzle_bits_wide = 6;
+int g_tile_mfma = 1;
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -761,6 +762,8 @@ int get_block_swizzle() { return g_swizzle_bits; }
 void set_block_swizzle_wide(int bits) { g_swizzle_bits_wide = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
 int get_block_swizzle_wide() { return g_swizzle_bits_wide; }
 int get_block_swizzle_for(int k) { return k >= 4 ? g_swizzle_bits_wide : g_swizzle_bits; }
+void set_tile_mfma(int on) { g_tile_mfma = on ? 1 : 0; }
+int get_tile_mfma() { return g_tile_mfma; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -57,3 +57,26 @@ def test_product_does_not_import_oracle():
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "liboracle" not in text and "_ref/" not in text, f
+
+
+def test_tuning_parameters_round_trip():
+    """Every tuning parameter reads back what was set; out-of-range values
+    and unknown names raise and leave the setting as it was."""
+    import pytest
+    from hiqsimulator_amd import _cppsim_amd as m
+    ranges = {"tile_high_target": (0, 64), "block_swizzle": (0, 16),
+              "block_swizzle_wide": (0, 16), "tile_mfma": (0, 1)}
+    for name, (lo, hi) in ranges.items():
+        old = m.get_tuning(name)
+        try:
+            for v in (lo, hi):
+                m.set_tuning(name, v)
+                assert m.get_tuning(name) == v
+            for bad in (lo - 1, hi + 1):
+                with pytest.raises(RuntimeError):
+                    m.set_tuning(name, bad)
+                assert m.get_tuning(name) == hi
+        finally:
+            m.set_tuning(name, old)
+    with pytest.raises(RuntimeError):
+        m.get_tuning("no_such_knob")

@@ -40,6 +40,34 @@ def test_tiled_c128_matches_oracle(engine_mod, oracle_mod, n, op_width):
     assert np.abs(sim.get_state() - ref).max() <= 1e-12
 
 
+@pytest.mark.parametrize("mfma", [1, 0])
+@pytest.mark.parametrize("op_width", [3, 4, 5])
+def test_tile_ops_on_matrix_cores_and_vector_units(engine_mod, oracle_mod, mfma, op_width):
+    """Ops of at least 3 qubits on the matrix cores (mfma = 1) or every op
+    on the vector units (0): complex128 equals the oracle to 1e-12 and
+    complex64 keeps fidelity 1 - 1e-6."""
+    import hiqsimulator_amd
+    mod = hiqsimulator_amd._cppsim_amd
+    old = mod.get_tuning("tile_mfma")
+    n = 15
+    g = circuit(n, 10, 300 + op_width)
+    ref = oracle_state(oracle_mod, n, g).get_state()
+    print("tile ops mfma=%d op_width=%d" % (mfma, op_width), flush=True)
+    try:
+        mod.set_tuning("tile_mfma", mfma)
+        for prec in ("c128", "c64"):
+            sim = engine_mod.SimulatorMPI(1, n, 5, prec)
+            sim.allocate_qureg(list(range(n)), 0)
+            sim.apply_tiled(g, op_width)
+            got = sim.get_state()
+            if prec == "c128":
+                assert np.abs(got - ref).max() <= 1e-12
+            else:
+                assert fidelity(ref, got) >= 1 - 1e-6
+    finally:
+        mod.set_tuning("tile_mfma", old)
+
+
 @pytest.mark.skip(reason="killed for silence on MI355X (apply_tiled after permute_qubits with "
                          "gates pending); the host plan, the permutation and the tile-pass "
                          "mappings of this case check out in CPU emulation and the cause is "
