@@ -99,56 +99,68 @@ __device__ __forceinline__ void wg_pass(VT<R>* psi, uint32_t n, const VT<R>* mat
      }
 }
 
-// The same pass with the matrix read from global memory at a uniform
-// address (scalar loads into SGPRs, shared by the whole wave), two rows at a
-// time: no LDS traffic for the matrix, and no SGPR spills (a fully unrolled
-// row loop would hold the whole matrix).
-template <int K, typename R, int NT>
-__device__ __forceinline__ void wg_pass_gm(VT<R>* psi, uint32_t n, const VT<R>* __restrict__ mat,
-                                           const uint8_t (&pos)[5])
+// Offset of amplitude j of a group: bit l of j deposited at pos[l].
+template <int K>
+__device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
+{
+     uint32_t o = 0;
+#pragma unroll
+     for (int l = 0; l < K; ++l)
+          if ((j >> l) & 1) o |= 1u << pos[l];
+     return o;
+}
+
+// The same pass on a 2^N-amplitude LDS tile with the matrix read from
+// global memory at a uniform address (scalar loads into SGPRs, shared by the
+// whole wave), two rows at a time: no LDS traffic for the matrix, and no
+// SGPR spills (a fully unrolled row loop would hold the whole matrix).  When
+// the groups outnumber the threads, each thread reads all of its groups
+// before its first write, so their LDS latencies overlap.
+template <int K, typename R, int NT, int N = kTileBits>
+__device__ __forceinline__ void wg_pass_gm(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
 {
      using V = VT<R>;
      constexpr int D = 1 << K;
-     const uint32_t ng = 1u << (n - K);
+     constexpr int NG = 1 << (N - K);
+     constexpr int G = NG >= NT ? NG / NT : 1;  // groups per thread
+     static_assert(NG < NT || NG % NT == 0, "groups must divide over the threads");
      uint32_t off[D];
 #pragma unroll
-     for (int j = 0; j < D; ++j) {
-          uint32_t o = 0;
+     for (int j = 0; j < D; ++j) off[j] = amp_offset<K>(j, pos);
+     const uint32_t t0 = threadIdx.x;
+     if (NG < NT && t0 >= static_cast<uint32_t>(NG)) return;
+     uint32_t base[G];
+     V v[G][D];
 #pragma unroll
-          for (int l = 0; l < K; ++l)
-               if ((j >> l) & 1) o |= 1u << pos[l];
-          off[j] = o;
-     }
-     for (uint32_t t = threadIdx.x; t < ng; t += NT) {
-          uint32_t b = t;
+     for (int g = 0; g < G; ++g) {
+          uint32_t b = t0 + static_cast<uint32_t>(g) * NT;
 #pragma unroll
           for (int l = 0; l < K; ++l) {
                const uint32_t lo = b & ((1u << pos[l]) - 1u);
                b = ((b ^ lo) << 1) | lo;
           }
-          V v[D];
+          base[g] = b;
 #pragma unroll
-          for (int j = 0; j < D; ++j) v[j] = psi[b + off[j]];
+          for (int j = 0; j < D; ++j) v[g][j] = psi[b + off[j]];
+     }
 #pragma unroll 2
-          for (int i = 0; i < D; ++i) {
-               const V* const row = mat + i * D;
+     for (int i = 0; i < D; ++i) {
+          const V* const row = mat + i * D;
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
                R re = 0, im = 0;
 #pragma unroll
                for (int j = 0; j < D; ++j) {
                     const V m = row[j];
-                    re = fma(m.x, v[j].x, re);
-                    re = fma(-m.y, v[j].y, re);
-                    im = fma(m.x, v[j].y, im);
-                    im = fma(m.y, v[j].x, im);
+                    re = fma(m.x, v[g][j].x, re);
+                    re = fma(-m.y, v[g][j].y, re);
+                    im = fma(m.x, v[g][j].y, im);
+                    im = fma(m.y, v[g][j].x, im);
                }
-               uint32_t oi = 0;
-#pragma unroll
-               for (int l = 0; l < K; ++l)
-                    if ((i >> l) & 1) oi |= 1u << pos[l];
                V o;
                o.x = re;
                o.y = im;
-               psi[b + oi] = o;
+               psi[base[g] + off[i]] = o;
           }
      }
 }
@@ -197,19 +209,9 @@ struct Mfma16<float> {
      static __device__ __forceinline__ bool row_im(int r) { return r & 1; }
 };
 
-template <int K>
-__device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
-{
-     uint32_t o = 0;
-#pragma unroll
-     for (int l = 0; l < K; ++l)
-          if ((j >> l) & 1) o |= 1u << pos[l];
-     return o;
-}
 
-template <int K, typename R, int NT>
-__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, uint32_t n, const VT<R>* __restrict__ mat,
-                                             const uint8_t (&pos)[5])
+template <int K, typename R, int NT, int N = kTileBits>
+__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
 {
      static_assert(K >= 3 && K <= 5, "MFMA pass needs 3 <= K <= 5");
      using V = VT<R>;
@@ -219,9 +221,11 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, uint32_t n, const VT<R>
      constexpr int RB = D / 8;  // 16-row blocks of the real output
      constexpr int T = D / 4;   // complex input amplitudes per lane
      constexpr int NW = NT / 64;
+     constexpr int NCB = (1 << (N - K)) / 16;  // 16-group column blocks
+     static_assert(NCB % NW == 0, "column blocks must divide over the waves");
+     constexpr int CB = NCB / NW;  // column blocks per wave
      constexpr bool kHoldA = RB * T <= 8;
-     const uint32_t ncb = (1u << (n - K)) >> 4;
-     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
      const int q = lane >> 4, c = lane & 15;
      // A fragments: lane (q, c) holds row c of block rb (output amplitude
      // 8*rb + row_amp(c), part row_im(c)) at input amplitudes 4t + q.
@@ -240,34 +244,45 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, uint32_t n, const VT<R>
 #pragma unroll
           for (int rb = 0; rb < RB; ++rb) load_a(rb, ahold[rb]);
      }
-     uint32_t in_off[T];
+     uint32_t in_off[T], out_off[RB][2];
 #pragma unroll
      for (int t = 0; t < T; ++t) in_off[t] = amp_offset<K>(4 * t + q, pos);
-     for (uint32_t cb = w; cb < ncb; cb += NW) {
-          uint32_t b = cb * 16 + c;
+#pragma unroll
+     for (int rb = 0; rb < RB; ++rb) {
+          out_off[rb][0] = amp_offset<K>(rb * 8 + 2 * q, pos);
+          out_off[rb][1] = amp_offset<K>(rb * 8 + 2 * q + 1, pos);
+     }
+     // every amplitude of the wave's column blocks is read before the first write
+     uint32_t base[CB];
+     V x[CB][T];
+#pragma unroll
+     for (int i = 0; i < CB; ++i) {
+          uint32_t b = static_cast<uint32_t>(w + i * NW) * 16 + c;
 #pragma unroll
           for (int l = 0; l < K; ++l) {
                const uint32_t lo = b & ((1u << pos[l]) - 1u);
                b = ((b ^ lo) << 1) | lo;
           }
-          V x[T];
+          base[i] = b;
 #pragma unroll
-          for (int t = 0; t < T; ++t) x[t] = psi[b + in_off[t]];
+          for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
+     }
+#pragma unroll
+     for (int i = 0; i < CB; ++i) {
           auto block = [&](const R (&a)[2 * T], int rb) {
                acc_t acc = {0, 0, 0, 0};
 #pragma unroll
                for (int t = 0; t < T; ++t) {
-                    acc = M::step(a[2 * t], x[t].x, acc);
-                    acc = M::step(a[2 * t + 1], x[t].y, acc);
+                    acc = M::step(a[2 * t], x[i][t].x, acc);
+                    acc = M::step(a[2 * t + 1], x[i][t].y, acc);
                }
-               const int o = rb * 8 + 2 * q;
                V y0, y1;
                y0.x = acc[0];
                y0.y = acc[1];
                y1.x = acc[2];
                y1.y = acc[3];
-               psi[b + amp_offset<K>(o, pos)] = y0;
-               psi[b + amp_offset<K>(o + 1, pos)] = y1;
+               psi[base[i] + out_off[rb][0]] = y0;
+               psi[base[i] + out_off[rb][1]] = y1;
           };
 #pragma unroll
           for (int rb = 0; rb < RB; ++rb) {
@@ -475,13 +490,14 @@ struct MatStage {
 
 // Waves per SIMD the tile kernels are built for: two 512-thread workgroups
 // per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
-// VGPRs), three for complex64 (48 KiB; at most 80 VGPRs); complex128
-// programs with 5-qubit ops are left unconstrained (their 5-qubit pass
-// needs more registers).
+// VGPRs), three for complex64 (48 KiB; at most 80 VGPRs).  Programs with
+// 5-qubit ops get more registers (complex128 unconstrained, complex64 two
+// workgroups per CU): their 5-qubit passes need them.
 template <typename R, int KMAX>
 constexpr int tile_waves_per_eu()
 {
-     return sizeof(R) == 8 ? (KMAX >= 5 ? 1 : 4) : 6;
+     if (sizeof(R) == 8) return KMAX >= 5 ? 1 : 4;
+     return KMAX >= 5 ? 4 : 6;
 }
 
 // ops[0..nops) on the tile in LDS, a workgroup barrier before each.  The
@@ -503,15 +519,15 @@ __device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const i
                if (o.k >= MK) {
                     if (in_lds) {
                          const V* const m = smat + soff;
-                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, kTileBits, m, o.rpos);
-                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, kTileBits, m, o.rpos);
-                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, kTileBits, m, o.rpos);
+                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, m, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, m, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, m, o.rpos);
                     }
                     else {
                          const V* const m = mats + o.mat_off;
-                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, kTileBits, m, o.rpos);
-                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, kTileBits, m, o.rpos);
-                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, kTileBits, m, o.rpos);
+                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, m, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, m, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, m, o.rpos);
                     }
                     soff += sz;
                     continue;
@@ -520,16 +536,16 @@ __device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const i
           soff += sz;
           const V* const m = mats + o.mat_off;
           switch (o.k) {
-               case 1: wg_pass_gm<1, R, NT>(tile, kTileBits, m, o.rpos); break;
-               case 2: wg_pass_gm<2, R, NT>(tile, kTileBits, m, o.rpos); break;
+               case 1: wg_pass_gm<1, R, NT>(tile, m, o.rpos); break;
+               case 2: wg_pass_gm<2, R, NT>(tile, m, o.rpos); break;
                case 3:
-                    if constexpr (MK > 3) wg_pass_gm<3, R, NT>(tile, kTileBits, m, o.rpos);
+                    if constexpr (MK > 3) wg_pass_gm<3, R, NT>(tile, m, o.rpos);
                     break;
                case 4:
-                    if constexpr (MK > 4) wg_pass_gm<4, R, NT>(tile, kTileBits, m, o.rpos);
+                    if constexpr (MK > 4) wg_pass_gm<4, R, NT>(tile, m, o.rpos);
                     break;
                default:
-                    if constexpr (KMAX >= 5 && MK > 5) wg_pass_gm<5, R, NT>(tile, kTileBits, m, o.rpos);
+                    if constexpr (KMAX >= 5 && MK > 5) wg_pass_gm<5, R, NT>(tile, m, o.rpos);
                     break;
           }
      }
