@@ -105,51 +105,61 @@ std::vector<PlannedPass> plan_circuit(const std::vector<BatchGate>& gates, int k
      return out;
 }
 
-std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width)
+PlannedTile fuse_tile(const std::vector<BatchGate>& gates, const TilePlan& tp)
 {
-     std::vector<PlannedTile> out;
-     if (gates.empty()) return out;
+     PlannedTile t;
+     int rank[64];
+     for (int p = 0; p < 64; ++p) rank[p] = p < 6 ? p : -1;
+     if (tp.hpos.size() != static_cast<size_t>(kTileHigh)) throw std::runtime_error("batch: bad tile plan");
+     for (int j = 0; j < kTileHigh; ++j) {
+          t.hpos[j] = static_cast<uint8_t>(tp.hpos[j]);
+          rank[tp.hpos[j]] = 6 + j;
+     }
+     for (const auto& op: tp.ops) {
+          Fusion f;
+          for (int gi: op) {
+               const BatchGate& g = gates[gi];
+               Mat m(size_t(1) << g.targets.size());
+               m.a = g.m;
+               std::vector<int64_t> ids = g.targets;
+               add_controls(m, ids, g.ctrls);
+               f.insert(std::move(m), false, std::move(ids), {});
+          }
+          Mat M;
+          std::vector<int64_t> ids, ctrls;
+          bool diag = false;
+          f.perform(M, ids, ctrls, diag);
+          if (ids.size() > static_cast<size_t>(kMaxWidth) || !ctrls.empty())
+               throw std::runtime_error("batch: a fused op spans more than 5 qubits");
+          PlannedPass p;
+          p.k = static_cast<int>(ids.size());
+          for (int l = 0; l < p.k; ++l) {
+               if (rank[ids[l]] < 0) throw std::runtime_error("batch: op outside its tile");
+               p.pos[l] = static_cast<uint8_t>(rank[ids[l]]);
+          }
+          p.m = std::move(M.a);
+          t.ops.push_back(std::move(p));
+     }
+     return t;
+}
+
+GateList gate_list(const std::vector<BatchGate>& gates)
+{
      GateList gl;
      for (const auto& g: gates) {
           gl.gate.push_back(g.targets);
           gl.ctrl.push_back(g.ctrls);
           gl.diag.push_back(false);
      }
-     for (const TilePlan& tp: plan_tile_passes(gl, n, kTileHigh, op_width)) {
-          PlannedTile t;
-          int rank[64];
-          for (int p = 0; p < 64; ++p) rank[p] = p < 6 ? p : -1;
-          for (int j = 0; j < kTileHigh; ++j) {
-               t.hpos[j] = static_cast<uint8_t>(tp.hpos[j]);
-               rank[tp.hpos[j]] = 6 + j;
-          }
-          for (const auto& op: tp.ops) {
-               Fusion f;
-               for (int gi: op) {
-                    const BatchGate& g = gates[gi];
-                    Mat m(size_t(1) << g.targets.size());
-                    m.a = g.m;
-                    std::vector<int64_t> ids = g.targets;
-                    add_controls(m, ids, g.ctrls);
-                    f.insert(std::move(m), false, std::move(ids), {});
-               }
-               Mat M;
-               std::vector<int64_t> ids, ctrls;
-               bool diag = false;
-               f.perform(M, ids, ctrls, diag);
-               if (ids.size() > static_cast<size_t>(kMaxWidth) || !ctrls.empty())
-                    throw std::runtime_error("batch: a fused op spans more than 5 qubits");
-               PlannedPass p;
-               p.k = static_cast<int>(ids.size());
-               for (int l = 0; l < p.k; ++l) {
-                    if (rank[ids[l]] < 0) throw std::runtime_error("batch: op outside its tile");
-                    p.pos[l] = static_cast<uint8_t>(rank[ids[l]]);
-               }
-               p.m = std::move(M.a);
-               t.ops.push_back(std::move(p));
-          }
-          out.push_back(std::move(t));
-     }
+     return gl;
+}
+
+std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width)
+{
+     std::vector<PlannedTile> out;
+     if (gates.empty()) return out;
+     for (const TilePlan& tp: plan_tile_passes(gate_list(gates), n, kTileHigh, op_width))
+          out.push_back(fuse_tile(gates, tp));
      return out;
 }
 

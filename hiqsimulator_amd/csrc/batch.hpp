@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "sched.hpp"
 
 namespace hq {
 
@@ -207,6 +208,10 @@ struct PlannedTile {
 std::vector<PlannedPass> plan_circuit(const std::vector<BatchGate>& gates, int k);
 // Tile passes (plan_tile_passes) with ops of at most op_width qubits fused.
 std::vector<PlannedTile> plan_circuit_tiles(const std::vector<BatchGate>& gates, int n, int op_width);
+// One planned tile pass (gate indices into `gates`) with its ops fused.
+PlannedTile fuse_tile(const std::vector<BatchGate>& gates, const TilePlan& tp);
+// The gates' positions as the schedulers take them.
+GateList gate_list(const std::vector<BatchGate>& gates);
 
 struct BatchStats {
      uint64_t circuits = 0;

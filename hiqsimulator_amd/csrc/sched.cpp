@@ -509,111 +509,125 @@ std::vector<int> plan_clusters(const GateList& g, int cluster_size, int mode)
      return cluster_of;
 }
 
-std::vector<TilePlan> plan_tile_passes(const GateList& g, int n, int tile_high, int op_width)
+TilePassPlanner::TilePassPlanner(const GateList& g, int n, int tile_high, int op_width)
+    : g_(g), n_(n), tile_high_(tile_high), op_width_(op_width)
 {
-     check_sizes(g);
+     check_sizes(g_);
      if (n < 6 + tile_high || n > 63) throw std::runtime_error("plan_tile_passes(): need 6 + tile_high <= n <= 63");
      if (tile_high < 1 || op_width < 1) throw std::runtime_error("plan_tile_passes(): bad tile or op width");
-     const size_t ng = g.gate.size();
-     std::vector<msk_t> q(ng);
+     const size_t ng = g_.gate.size();
+     q_.resize(ng);
      for (size_t i = 0; i < ng; ++i) {
           msk_t m = 0;
-          for (auto p: g.gate[i]) m |= msk_t(1) << p;
-          for (auto p: g.ctrl[i]) m |= msk_t(1) << p;
-          for (auto p: g.gate[i])
+          for (auto p: g_.gate[i]) m |= msk_t(1) << p;
+          for (auto p: g_.ctrl[i]) m |= msk_t(1) << p;
+          for (auto p: g_.gate[i])
                if (p < 0 || p >= n) throw std::runtime_error("plan_tile_passes(): position out of range");
-          for (auto p: g.ctrl[i])
+          for (auto p: g_.ctrl[i])
                if (p < 0 || p >= n) throw std::runtime_error("plan_tile_passes(): position out of range");
           if (popc(m & ~msk_t(63)) > tile_high) throw std::runtime_error("plan_tile_passes(): a gate does not fit the tile");
-          q[i] = m;
+          q_[i] = m;
      }
+     remaining_.resize(ng);
+     for (size_t i = 0; i < ng; ++i) remaining_[i] = i;
+}
+
+// Gates of the remaining list a tile over S would take.
+int TilePassPlanner::score(uint64_t S) const
+{
+     msk_t blocked = 0;
+     int cnt = 0;
+     for (size_t j = 0; j < remaining_.size(); ++j) {
+          const msk_t m = q_[remaining_[j]];
+          if (!(m & blocked) && submask(m, S)) ++cnt;
+          else blocked |= m;
+          if (submask(S, blocked)) break;
+     }
+     return cnt;
+}
+
+bool TilePassPlanner::next(TilePlan& out)
+{
+     if (remaining_.empty()) return false;
      const msk_t kFree = 63;
-     std::vector<size_t> remaining(ng);
-     for (size_t i = 0; i < ng; ++i) remaining[i] = i;
-     auto score = [&](msk_t S) {
-          msk_t blocked = 0;
-          int cnt = 0;
-          for (size_t j = 0; j < remaining.size(); ++j) {
-               const msk_t m = q[remaining[j]];
-               if (!(m & blocked) && submask(m, S)) ++cnt;
-               else blocked |= m;
-               if (submask(S, blocked)) break;
-          }
-          return cnt;
-     };
      const size_t kWindow = 64;
-     std::vector<TilePlan> out;
-     while (!remaining.empty()) {
-          // pool: high positions of the leading gates, in order of first use
-          std::vector<int> pool;
-          msk_t seen = 0;
-          for (size_t j = 0; j < remaining.size() && j < kWindow; ++j)
-               for (msk_t m = q[remaining[j]] & ~kFree & ~seen; m; m &= m - 1) {
-                    pool.push_back(__builtin_ctzll(m));
-                    seen |= m & (~m + 1);
-               }
-          msk_t H = q[remaining[0]] & ~kFree;  // the first gate always fits
-          int best = score(kFree | H);
-          while (popc(H) < tile_high) {
-               int add = -1, add_s = -1;
-               for (int p: pool) {
-                    if ((H >> p) & 1) continue;
-                    const int s = score(kFree | H | (msk_t(1) << p));
-                    if (s > add_s) {
-                         add_s = s;
-                         add = p;
-                    }
-               }
-               if (add < 0) break;
-               H |= msk_t(1) << add;
-               best = add_s;
+     // pool: high positions of the leading gates, in order of first use
+     std::vector<int> pool;
+     msk_t seen = 0;
+     for (size_t j = 0; j < remaining_.size() && j < kWindow; ++j)
+          for (msk_t m = q_[remaining_[j]] & ~kFree & ~seen; m; m &= m - 1) {
+               pool.push_back(__builtin_ctzll(m));
+               seen |= m & (~m + 1);
           }
-          // one round of single swaps
-          for (msk_t out_b = H; out_b; out_b &= out_b - 1) {
-               const msk_t ob = out_b & (~out_b + 1);
-               for (int p: pool) {
-                    if ((H >> p) & 1) continue;
-                    const msk_t H2 = (H & ~ob) | (msk_t(1) << p);
-                    const int s = score(kFree | H2);
-                    if (s > best) {
-                         best = s;
-                         H = H2;
-                         break;
-                    }
+     msk_t H = q_[remaining_[0]] & ~kFree;  // the first gate always fits
+     int best = score(kFree | H);
+     while (popc(H) < tile_high_) {
+          int add = -1, add_s = -1;
+          for (int p: pool) {
+               if ((H >> p) & 1) continue;
+               const int s = score(kFree | H | (msk_t(1) << p));
+               if (s > add_s) {
+                    add_s = s;
+                    add = p;
                }
           }
-          const msk_t S = kFree | H;
-          // taken gates
-          msk_t blocked = 0;
-          std::vector<size_t> taken, next;
-          for (size_t j = 0; j < remaining.size(); ++j) {
-               const msk_t m = q[remaining[j]];
-               if (!(m & blocked) && submask(m, S)) taken.push_back(remaining[j]);
-               else {
-                    blocked |= m;
-                    next.push_back(remaining[j]);
-               }
-          }
-          if (taken.empty()) throw std::runtime_error("plan_tile_passes(): no gate could be scheduled");
-          remaining.swap(next);
-          // pad the high positions to tile_high with the lowest free ones
-          for (int p = 6; p < n && popc(H) < tile_high; ++p) H |= msk_t(1) << p;
-          TilePlan tp;
-          for (int p = 6; p < n; ++p)
-               if ((H >> p) & 1) tp.hpos.push_back(p);
-          // ops: lookahead packing of the taken gates
-          GateList sub;
-          for (size_t i: taken) {
-               sub.gate.push_back(g.gate[i]);
-               sub.ctrl.push_back(g.ctrl[i]);
-               sub.diag.push_back(false);
-          }
-          const std::vector<int> cl = plan_clusters(sub, op_width, kPlanLookahead);
-          const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
-          tp.ops.assign(nc, {});
-          for (size_t j = 0; j < taken.size(); ++j) tp.ops[cl[j]].push_back(static_cast<int>(taken[j]));
-          out.push_back(std::move(tp));
+          if (add < 0) break;
+          H |= msk_t(1) << add;
+          best = add_s;
      }
+     // one round of single swaps
+     for (msk_t out_b = H; out_b; out_b &= out_b - 1) {
+          const msk_t ob = out_b & (~out_b + 1);
+          for (int p: pool) {
+               if ((H >> p) & 1) continue;
+               const msk_t H2 = (H & ~ob) | (msk_t(1) << p);
+               const int s = score(kFree | H2);
+               if (s > best) {
+                    best = s;
+                    H = H2;
+                    break;
+               }
+          }
+     }
+     const msk_t S = kFree | H;
+     // taken gates
+     msk_t blocked = 0;
+     std::vector<size_t> taken, next;
+     for (size_t j = 0; j < remaining_.size(); ++j) {
+          const msk_t m = q_[remaining_[j]];
+          if (!(m & blocked) && submask(m, S)) taken.push_back(remaining_[j]);
+          else {
+               blocked |= m;
+               next.push_back(remaining_[j]);
+          }
+     }
+     if (taken.empty()) throw std::runtime_error("plan_tile_passes(): no gate could be scheduled");
+     remaining_.swap(next);
+     // pad the high positions to tile_high with the lowest free ones
+     for (int p = 6; p < n_ && popc(H) < tile_high_; ++p) H |= msk_t(1) << p;
+     out = TilePlan{};
+     for (int p = 6; p < n_; ++p)
+          if ((H >> p) & 1) out.hpos.push_back(p);
+     // ops: lookahead packing of the taken gates
+     GateList sub;
+     for (size_t i: taken) {
+          sub.gate.push_back(g_.gate[i]);
+          sub.ctrl.push_back(g_.ctrl[i]);
+          sub.diag.push_back(false);
+     }
+     const std::vector<int> cl = plan_clusters(sub, op_width_, kPlanLookahead);
+     const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
+     out.ops.assign(nc, {});
+     for (size_t j = 0; j < taken.size(); ++j) out.ops[cl[j]].push_back(static_cast<int>(taken[j]));
+     return true;
+}
+
+std::vector<TilePlan> plan_tile_passes(const GateList& g, int n, int tile_high, int op_width)
+{
+     TilePassPlanner planner(g, n, tile_high, op_width);
+     std::vector<TilePlan> out;
+     TilePlan tp;
+     while (planner.next(tp)) out.push_back(std::move(tp));
      return out;
 }
 

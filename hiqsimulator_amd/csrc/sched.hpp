@@ -63,6 +63,23 @@ struct TilePlan {
 };
 std::vector<TilePlan> plan_tile_passes(const GateList& g, int n, int tile_high, int op_width);
 
+// The same plan one tile pass at a time (plan_tile_passes is its loop), so
+// that a caller can start on the first passes while later ones are planned.
+class TilePassPlanner {
+public:
+     TilePassPlanner(const GateList& g, int n, int tile_high, int op_width);
+     // The next tile pass into `out`; false when every gate is planned.
+     bool next(TilePlan& out);
+     bool done() const { return remaining_.empty(); }
+
+private:
+     int score(uint64_t S) const;
+     GateList g_;
+     int n_, tile_high_, op_width_;
+     std::vector<uint64_t> q_;
+     std::vector<size_t> remaining_;
+};
+
 // swap_scheduler.cpp:70-88.  Returns the qubit ids to be local.
 std::vector<int64_t> schedule_swap(const GateList& g, int64_t num_splits, int num_locals,
                                    bool fuse);

@@ -94,10 +94,19 @@ Server::Server(Prec prec, int device, uint64_t state_bytes, uint64_t matrix_byte
      mat_pool_ = RangeAlloc(mat_entries);
      mat_cap_entries_ = mat_entries;
      for (auto& s: slots_) check(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
+     for (int i = 0; i < kPlanners; ++i) planners_.emplace_back([this] { planner_loop(); });
 }
 
 Server::~Server()
 {
+     {
+          std::lock_guard<std::mutex> lk(qmu_);
+          stop_ = true;
+          for (auto& kv: reqs_)
+               if (kv.second.plan) kv.second.plan->cancel = true;
+     }
+     qcv_.notify_all();
+     for (auto& t: planners_) t.join();
      (void)hipSetDevice(device_);
      (void)hipStreamSynchronize(stream_);
      for (auto& s: slots_) {
@@ -172,31 +181,18 @@ int64_t Server::submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint
      r.zmask = std::move(zmask);
      r.t_submit = now_s();
      const size_t ab = amp_bytes(prec_);
-     if (!gates.empty()) {
+     std::shared_ptr<PlanState> ps;
+     if (!gates.empty() && n >= std::max(prog_min_qubits_, kTileBits)) {
+          ps = std::make_shared<PlanState>();
+          ps->gates = std::move(gates);
+          ps->n = n;
+          ps->width = fusion_width(n);
+          r.plan = ps;
+     }
+     else if (!gates.empty()) {
           std::vector<std::vector<cplx>*> mats;
           std::vector<PlannedPass> fused;
-          std::vector<PlannedTile> tiles;
-          if (n >= std::max(prog_min_qubits_, kTileBits)) {
-               tiles = plan_circuit_tiles(gates, n, fusion_width(n));
-               for (auto& t: tiles) {
-                    TPass tp{};
-                    std::memcpy(tp.hpos, t.hpos, kTileHigh);
-                    tp.op_first = static_cast<uint32_t>(r.tops.size());
-                    tp.nops = static_cast<uint32_t>(t.ops.size());
-                    for (auto& op: t.ops) {
-                         TileOp to{};
-                         to.mat_off = r.mat_entries;
-                         to.k = static_cast<uint8_t>(op.k);
-                         std::memcpy(to.rpos, op.pos, 5);
-                         r.kmax_op = std::max(r.kmax_op, op.k);
-                         r.tops.push_back(to);
-                         r.mat_entries += op.m.size();
-                         mats.push_back(&op.m);
-                    }
-                    r.tpasses.push_back(tp);
-               }
-          }
-          else {
+          {
                fused = plan_circuit(gates, fusion_width(n));
                for (auto& p: fused) {
                     Pass q{p.k, {0, 0, 0, 0, 0}};
@@ -229,7 +225,77 @@ int64_t Server::submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint
      reqs_.emplace(id, std::move(r));
      waiting_.push_back(id);
      ++stats_.submitted;
+     if (ps) {
+          {
+               std::lock_guard<std::mutex> lk(qmu_);
+               plan_queue_.push_back(ps);
+          }
+          qcv_.notify_one();
+     }
      return id;
+}
+
+// Planner thread: plans queued tile-program requests pass by pass and
+// publishes each pass (ops and matrices) under the request's lock.
+void Server::planner_loop()
+{
+     const size_t ab = amp_bytes(prec_);
+     for (;;) {
+          std::shared_ptr<PlanState> ps;
+          {
+               std::unique_lock<std::mutex> lk(qmu_);
+               qcv_.wait(lk, [&] { return stop_ || !plan_queue_.empty(); });
+               if (stop_) return;
+               ps = plan_queue_.front();
+               plan_queue_.pop_front();
+          }
+          try {
+               TilePassPlanner planner(gate_list(ps->gates), ps->n, kTileHigh, ps->width);
+               TilePlan tp;
+               while (!ps->cancel && planner.next(tp)) {
+                    const PlannedTile t = fuse_tile(ps->gates, tp);
+                    size_t entries = 0;
+                    for (const auto& op: t.ops) entries += op.m.size();
+                    std::vector<char> mb(entries * ab);
+                    size_t o = 0;
+                    for (const auto& op: t.ops)
+                         for (size_t i = 0; i < op.m.size(); ++i, ++o) {
+                              if (prec_ == Prec::C128) {
+                                   double* d = reinterpret_cast<double*>(mb.data()) + 2 * o;
+                                   d[0] = op.m[i].real();
+                                   d[1] = op.m[i].imag();
+                              }
+                              else {
+                                   float* d = reinterpret_cast<float*>(mb.data()) + 2 * o;
+                                   d[0] = static_cast<float>(op.m[i].real());
+                                   d[1] = static_cast<float>(op.m[i].imag());
+                              }
+                         }
+                    std::lock_guard<std::mutex> g(ps->m);
+                    TPass pass{};
+                    std::memcpy(pass.hpos, t.hpos, kTileHigh);
+                    pass.op_first = static_cast<uint32_t>(ps->tops.size());
+                    pass.nops = static_cast<uint32_t>(t.ops.size());
+                    for (const auto& op: t.ops) {
+                         TileOp to{};
+                         to.mat_off = ps->mat_entries;
+                         to.k = static_cast<uint8_t>(op.k);
+                         std::memcpy(to.rpos, op.pos, 5);
+                         ps->kmax_op = std::max(ps->kmax_op, op.k);
+                         ps->tops.push_back(to);
+                         ps->mat_entries += op.m.size();
+                    }
+                    ps->mats.insert(ps->mats.end(), mb.begin(), mb.end());
+                    ps->tpasses.push_back(pass);
+               }
+          }
+          catch (const std::exception& e) {
+               std::lock_guard<std::mutex> g(ps->m);
+               ps->error = e.what();
+          }
+          std::lock_guard<std::mutex> g(ps->m);
+          ps->done = true;
+     }
 }
 
 void Server::ensure_slot(Slot& s, size_t desc_bytes, size_t out_bytes)
@@ -305,6 +371,16 @@ std::vector<int64_t> Server::step(bool block)
      std::vector<int64_t> admitted;
      for (auto it = waiting_.begin(); it != waiting_.end() && active_.size() < max_active_;) {
           Req& r = reqs_.at(*it);
+          if (r.plan) {  // tile programs: admitted once the first pass is planned
+               std::lock_guard<std::mutex> g(r.plan->m);
+               if (!r.plan->error.empty())
+                    throw std::runtime_error("Server: planning request " + std::to_string(*it) +
+                                             " failed: " + r.plan->error);
+               if (r.plan->tpasses.empty() && !r.plan->done) {
+                    ++it;
+                    continue;
+               }
+          }
           uint64_t so = 0, mo = 0;
           if (!state_pool_.alloc(uint64_t(1) << r.n, 256 / ab, &so)) {
                ++it;
@@ -330,12 +406,6 @@ std::vector<int64_t> Server::step(bool block)
           if (block && !inflight_.empty()) harvest(true, completed);
           return completed;
      }
-     // A free slot: wait for the oldest step when both are in flight.
-     if (inflight_.size() >= static_cast<size_t>(kServeSlots)) harvest(true, completed);
-     const int si = next_slot_;
-     next_slot_ = (next_slot_ + 1) % kServeSlots;
-     Slot& slot = slots_[si];
-
      // Descriptors of this step.
      ItemList init;
      for (int64_t id: admitted) {
@@ -352,31 +422,52 @@ std::vector<int64_t> Server::step(bool block)
      std::vector<StepLayer> layers(pps_);
      int kmax = 1;
      std::vector<int64_t> finishing;
-     std::vector<TileOp> step_ops;  // ops of this step's tile programs (absolute matrix offsets)
+     std::vector<TileOp> step_ops;  // ops of this step's tile programs (matrix offsets into step_mats)
+     std::vector<char> step_mats;   // their matrices, shipped with the step's descriptors
+     uint64_t step_mat_entries = 0;
+     bool work = !init.items.empty();
      for (int64_t id: active_) {
           Req& r = reqs_.at(id);
-          const size_t total = r.tpasses.empty() ? r.passes.size() : r.tpasses.size();
-          for (int l = 0; l < pps_ && r.next + l < r.tpasses.size(); ++l) {
-               const TPass& tp = r.tpasses[r.next + l];
-               if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ || r.n < kTileBits)
-                    throw std::runtime_error("Server: internal bounds check failed");
-               BatchTileProg it{};
-               it.state_off = r.state_off;
-               it.op_first = static_cast<uint32_t>(step_ops.size());
-               it.nops = static_cast<uint16_t>(tp.nops);
-               it.log_blocks = static_cast<uint8_t>(r.n - kTileBits);
-               std::memcpy(it.hpos, tp.hpos, kTileHigh);
-               for (uint32_t o = tp.op_first; o < tp.op_first + tp.nops; ++o) {
-                    TileOp to = r.tops[o];
-                    to.mat_off += r.mat_off;
-                    if (to.mat_off + (uint64_t(1) << (2 * to.k)) > mat_cap_entries_)
+          r.adv = 0;
+          if (r.plan) {
+               PlanState& ps = *r.plan;
+               std::lock_guard<std::mutex> g(ps.m);
+               if (!ps.error.empty())
+                    throw std::runtime_error("Server: planning request " + std::to_string(id) + " failed: " + ps.error);
+               for (int l = 0; l < pps_ && r.next + l < ps.tpasses.size(); ++l) {
+                    const TPass& tp = ps.tpasses[r.next + l];
+                    if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ || r.n < kTileBits)
                          throw std::runtime_error("Server: internal bounds check failed");
-                    step_ops.push_back(to);
+                    BatchTileProg it{};
+                    it.state_off = r.state_off;
+                    it.op_first = static_cast<uint32_t>(step_ops.size());
+                    it.nops = static_cast<uint16_t>(tp.nops);
+                    it.log_blocks = static_cast<uint8_t>(r.n - kTileBits);
+                    std::memcpy(it.hpos, tp.hpos, kTileHigh);
+                    for (uint32_t o = tp.op_first; o < tp.op_first + tp.nops; ++o) {
+                         TileOp to = ps.tops[o];
+                         const uint64_t ne = uint64_t(1) << (2 * to.k);
+                         if ((to.mat_off + ne) * ab > ps.mats.size())
+                              throw std::runtime_error("Server: internal bounds check failed");
+                         step_mats.insert(step_mats.end(), ps.mats.begin() + to.mat_off * ab,
+                                          ps.mats.begin() + (to.mat_off + ne) * ab);
+                         to.mat_off = step_mat_entries;
+                         step_mat_entries += ne;
+                         step_ops.push_back(to);
+                    }
+                    layers[l].prog_kmax = std::max(layers[l].prog_kmax, ps.kmax_op);
+                    layers[l].progs.add(it, uint64_t(1) << it.log_blocks);
+                    ++r.adv;
+                    work = true;
                }
-               layers[l].prog_kmax = std::max(layers[l].prog_kmax, r.kmax_op);
-               layers[l].progs.add(it, uint64_t(1) << it.log_blocks);
+               if (ps.done && r.next + r.adv >= ps.tpasses.size()) {
+                    finishing.push_back(id);
+                    work = true;
+               }
+               continue;
           }
-          for (int l = 0; l < pps_ && r.tpasses.empty() && r.next + l < r.passes.size(); ++l) {
+          const size_t total = r.passes.size();
+          for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
                const Pass& p = r.passes[r.next + l];
                const uint64_t moff = r.mat_off + r.pass_mat[r.next + l];
                const bool tile_pays = p.pos[0] < get_tile_high_target() &&
@@ -409,8 +500,25 @@ std::vector<int64_t> Server::step(bool block)
                kmax = std::max(kmax, p.k);
                layers[l].direct.add(it, ((uint64_t(1) << it.log_groups) + kBatchLayerBlock - 1) / kBatchLayerBlock);
           }
+          r.adv = std::min(total - std::min(total, r.next), static_cast<size_t>(pps_));
           if (r.next + pps_ >= total) finishing.push_back(id);
+          work = true;
      }
+     if (!work) {
+          // Only tile-program requests whose next passes are still being
+          // planned: nothing to launch yet.
+          if (block) {
+               if (!inflight_.empty()) harvest(true, completed);
+               else std::this_thread::sleep_for(std::chrono::microseconds(200));
+          }
+          return completed;
+     }
+     // A free slot: wait for the oldest step when both are in flight (slots
+     // are used strictly in turn, so the oldest in flight is the next one).
+     if (inflight_.size() >= static_cast<size_t>(kServeSlots)) harvest(true, completed);
+     const int si = next_slot_;
+     next_slot_ = (next_slot_ + 1) % kServeSlots;
+     Slot& slot = slots_[si];
      // Outputs of the requests that finish in this step.
      ItemList sums;
      std::vector<SampleJob> jobs;
@@ -466,6 +574,8 @@ std::vector<int64_t> Server::step(bool block)
      }
      const size_t o_ops = o;
      o += align_up(step_ops.size() * sizeof(TileOp), 256);
+     const size_t o_tmats = o;
+     o += align_up(step_mats.size(), 256);
      const size_t o_sums = o;
      o += align_up(sums.items.size() * sizeof(BatchItem), 256);
      const size_t o_jobs = o;
@@ -474,6 +584,7 @@ std::vector<int64_t> Server::step(bool block)
      ensure_slot(slot, desc_bytes, std::max<size_t>(out_bytes, 256));
      std::memcpy(slot.h_desc + o_init, init.items.data(), init.items.size() * sizeof(BatchItem));
      std::memcpy(slot.h_desc + o_ops, step_ops.data(), step_ops.size() * sizeof(TileOp));
+     std::memcpy(slot.h_desc + o_tmats, step_mats.data(), step_mats.size());
      for (int l = 0; l < pps_; ++l) {
           std::memcpy(slot.h_desc + o_prog[l], layers[l].progs.items.data(),
                       layers[l].progs.items.size() * sizeof(BatchTileProg));
@@ -505,7 +616,7 @@ std::vector<int64_t> Server::step(bool block)
      }
      for (int l = 0; l < pps_; ++l) {
           if (!layers[l].progs.items.empty()) {
-               check(launch_batch_tprog(d_states_, prec_, d_mats_,
+               check(launch_batch_tprog(d_states_, prec_, slot.d_desc + o_tmats,
                                         reinterpret_cast<const BatchTileProg*>(slot.d_desc + o_prog[l]),
                                         static_cast<int>(layers[l].progs.items.size()),
                                         reinterpret_cast<const TileOp*>(slot.d_desc + o_ops),
@@ -565,13 +676,13 @@ std::vector<int64_t> Server::step(bool block)
      std::vector<int64_t> still;
      for (int64_t id: active_) {
           Req& r = reqs_.at(id);
-          const size_t total = r.tpasses.empty() ? r.passes.size() : r.tpasses.size();
-          r.next = std::min(total, r.next + static_cast<size_t>(pps_));
+          r.next += r.adv;
           if (std::find(finishing.begin(), finishing.end(), id) != finishing.end()) {
                r.state = 2;
                state_pool_.release(r.state_off, uint64_t(1) << r.n);
                if (r.mat_entries) mat_pool_.release(r.mat_off, r.mat_entries);
                std::vector<char>().swap(r.mats);
+               r.plan.reset();
           }
           else {
                still.push_back(id);

@@ -13,6 +13,10 @@
 //    them.  A request of at least prog_min_qubits (and 12) qubits runs as
 //    tile programs (several fused ops per HBM pass, applied in LDS); smaller
 //    ones as fused passes;
+//  * plans tile-program requests on planner threads, one tile pass at a
+//    time: submit() returns at once, the request is admitted as soon as its
+//    first pass is planned and each step runs the passes planned so far, so
+//    planning overlaps the GPU work of the same and of other requests;
 //  * when a request's last pass is in a step, its outputs are computed in
 //    the same step and copied back; its memory returns to the pools at
 //    once (the stream orders later use after the outputs);
@@ -24,10 +28,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <complex>
+#include <condition_variable>
 #include <cstdint>
 #include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "batch.hpp"
@@ -78,8 +88,24 @@ private:
      };
      struct TPass {
           uint8_t hpos[kTileHigh];
-          uint32_t op_first;  // into Req::tops
+          uint32_t op_first;  // into PlanState::tops
           uint32_t nops;
+     };
+     // Tile-program plan of one request, filled pass by pass by a planner
+     // thread; every field but `gates`, `n`, `width` and `cancel` is guarded
+     // by `m`.
+     struct PlanState {
+          std::vector<BatchGate> gates;
+          int n = 0, width = 4;
+          std::mutex m;
+          std::vector<TPass> tpasses;  // planned passes
+          std::vector<TileOp> tops;    // their ops, mat_off into mats (entries)
+          std::vector<char> mats;      // their matrices in the state precision
+          uint64_t mat_entries = 0;
+          int kmax_op = 1;
+          bool done = false;
+          std::string error;
+          std::atomic<bool> cancel{false};
      };
      struct Req {
           int n = 0;
@@ -89,9 +115,8 @@ private:
           std::vector<uint64_t> xmask, zmask;
           std::vector<Pass> passes;
           std::vector<uint64_t> pass_mat;  // first matrix entry of each pass (relative)
-          std::vector<TPass> tpasses;      // tile programs (instead of passes)
-          std::vector<TileOp> tops;        // their ops, mat_off relative
-          int kmax_op = 1;
+          std::shared_ptr<PlanState> plan;  // tile programs (instead of passes)
+          size_t adv = 0;                   // passes launched in the current step
           std::vector<char> mats;  // fused matrices in the state precision
           uint64_t mat_entries = 0;
           size_t next = 0;
@@ -114,6 +139,7 @@ private:
      };
 
      void harvest(bool wait_oldest, std::vector<int64_t>& completed);
+     void planner_loop();
      void ensure_slot(Slot& s, size_t desc_bytes, size_t out_bytes);
      int fusion_width(int n) const;
 
@@ -136,6 +162,12 @@ private:
      std::deque<int64_t> waiting_;
      std::vector<int64_t> active_;
      Slot slots_[kServeSlots];
+     static constexpr int kPlanners = 2;
+     std::vector<std::thread> planners_;
+     std::mutex qmu_;
+     std::condition_variable qcv_;
+     std::deque<std::shared_ptr<PlanState>> plan_queue_;
+     bool stop_ = false;
      std::deque<int> inflight_;
      int next_slot_ = 0;
      int64_t next_id_ = 0;
