@@ -252,47 +252,52 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict
           out_off[rb][0] = amp_offset<K>(rb * 8 + 2 * q, pos);
           out_off[rb][1] = amp_offset<K>(rb * 8 + 2 * q + 1, pos);
      }
-     // every amplitude of the wave's column blocks is read before the first write
-     uint32_t base[CB];
-     V x[CB][T];
+     // The wave's column blocks in groups of HB: every amplitude of a group
+     // is read before the group's first write.
+     constexpr int HB = CB < 2 ? CB : 2;
 #pragma unroll
-     for (int i = 0; i < CB; ++i) {
-          uint32_t b = static_cast<uint32_t>(w + i * NW) * 16 + c;
+     for (int i0 = 0; i0 < CB; i0 += HB) {
+          uint32_t base[HB];
+          V x[HB][T];
 #pragma unroll
-          for (int l = 0; l < K; ++l) {
-               const uint32_t lo = b & ((1u << pos[l]) - 1u);
-               b = ((b ^ lo) << 1) | lo;
+          for (int i = 0; i < HB; ++i) {
+               uint32_t b = static_cast<uint32_t>(w + (i0 + i) * NW) * 16 + c;
+#pragma unroll
+               for (int l = 0; l < K; ++l) {
+                    const uint32_t lo = b & ((1u << pos[l]) - 1u);
+                    b = ((b ^ lo) << 1) | lo;
+               }
+               base[i] = b;
+#pragma unroll
+               for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
           }
-          base[i] = b;
 #pragma unroll
-          for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
-     }
+          for (int i = 0; i < HB; ++i) {
+               auto block = [&](const R (&a)[2 * T], int rb) {
+                    acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-     for (int i = 0; i < CB; ++i) {
-          auto block = [&](const R (&a)[2 * T], int rb) {
-               acc_t acc = {0, 0, 0, 0};
+                    for (int t = 0; t < T; ++t) {
+                         acc = M::step(a[2 * t], x[i][t].x, acc);
+                         acc = M::step(a[2 * t + 1], x[i][t].y, acc);
+                    }
+                    V y0, y1;
+                    y0.x = acc[0];
+                    y0.y = acc[1];
+                    y1.x = acc[2];
+                    y1.y = acc[3];
+                    psi[base[i] + out_off[rb][0]] = y0;
+                    psi[base[i] + out_off[rb][1]] = y1;
+               };
 #pragma unroll
-               for (int t = 0; t < T; ++t) {
-                    acc = M::step(a[2 * t], x[i][t].x, acc);
-                    acc = M::step(a[2 * t + 1], x[i][t].y, acc);
-               }
-               V y0, y1;
-               y0.x = acc[0];
-               y0.y = acc[1];
-               y1.x = acc[2];
-               y1.y = acc[3];
-               psi[base[i] + out_off[rb][0]] = y0;
-               psi[base[i] + out_off[rb][1]] = y1;
-          };
-#pragma unroll
-          for (int rb = 0; rb < RB; ++rb) {
-               if constexpr (kHoldA) {
-                    block(ahold[rb], rb);
-               }
-               else {
-                    R a[2 * T];
-                    load_a(rb, a);
-                    block(a, rb);
+               for (int rb = 0; rb < RB; ++rb) {
+                    if constexpr (kHoldA) {
+                         block(ahold[rb], rb);
+                    }
+                    else {
+                         R a[2 * T];
+                         load_a(rb, a);
+                         block(a, rb);
+                    }
                }
           }
      }
