@@ -479,6 +479,7 @@ struct MatStage {
                }
                start += sz;
           }
+          if (start == 0) return 0;  // nothing staged (no ops): mats may be empty
 #pragma unroll
           for (int i = 0; i < PER; ++i) r[i] = mats[addr[i]];
           return start;
