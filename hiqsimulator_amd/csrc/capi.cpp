@@ -726,6 +726,14 @@ int hiqsim_server_set_prog_min_qubits(hiqsim_server_t* s, int q)
      });
 }
 
+int hiqsim_server_set_heavy_qubits(hiqsim_server_t* s, int q)
+{
+     return guard([&] {
+          need(s, "server");
+          s->s.set_heavy_qubits(q);
+     });
+}
+
 int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
                          const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,
                          const size_t* ctrl_off, int kind, uint32_t shots, uint64_t seed,

@@ -594,6 +594,7 @@ public:
           ok(hiqsim_server_set_widths(h_, v.data(), v.size()));
      }
      void set_prog_min_qubits(int q) { ok(hiqsim_server_set_prog_min_qubits(h_, q)); }
+     void set_heavy_qubits(int q) { ok(hiqsim_server_set_heavy_qubits(h_, q)); }
      int64_t submit_samples(int n, const py::list& gates, uint32_t shots, uint64_t seed)
      {
           const GateCsr g = gate_csr(gates);
@@ -768,6 +769,7 @@ PYBIND11_MODULE(_cppsim_amd, m)
               py::arg("passes_per_step") = 8, py::arg("max_active") = 1024)
          .def("set_widths", &PyServer::set_widths)
          .def("set_prog_min_qubits", &PyServer::set_prog_min_qubits)
+         .def("set_heavy_qubits", &PyServer::set_heavy_qubits)
          .def("submit_samples", &PyServer::submit_samples, py::arg("n_qubits"), py::arg("gates"),
               py::arg("shots") = 1024, py::arg("seed") = 0)
          .def("submit_expectation", &PyServer::submit_expectation, py::arg("n_qubits"),

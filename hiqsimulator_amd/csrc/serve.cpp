@@ -127,6 +127,12 @@ void Server::set_prog_min_qubits(int q)
      prog_min_qubits_ = q;
 }
 
+void Server::set_heavy_qubits(int q)
+{
+     if (q < 1 || q > 31) throw std::runtime_error("Server: heavy_qubits must be 1..31");
+     heavy_qubits_ = q;
+}
+
 void Server::set_widths(const std::vector<int>& k_by_n)
 {
      for (int k: k_by_n)
@@ -369,8 +375,16 @@ std::vector<int64_t> Server::step(bool block)
      // Admission in arrival order; a request that does not fit lets later
      // ones try.
      std::vector<int64_t> admitted;
+     bool heavy_active = false;
+     for (int64_t id: active_)
+          if (reqs_.at(id).n >= heavy_qubits_) heavy_active = true;
      for (auto it = waiting_.begin(); it != waiting_.end() && active_.size() < max_active_;) {
           Req& r = reqs_.at(*it);
+          const bool heavy = r.n >= heavy_qubits_;
+          if (heavy && heavy_active) {
+               ++it;
+               continue;
+          }
           if (r.plan) {  // tile programs: admitted once the first pass is planned
                std::lock_guard<std::mutex> g(r.plan->m);
                if (!r.plan->error.empty())
@@ -395,6 +409,7 @@ std::vector<int64_t> Server::step(bool block)
           r.mat_off = mo;
           r.state = 1;
           r.t_admit = now_s();
+          heavy_active = heavy_active || heavy;
           admitted.push_back(*it);
           active_.push_back(*it);
           it = waiting_.erase(it);

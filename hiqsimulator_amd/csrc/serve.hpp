@@ -6,7 +6,9 @@
 // expectation values of a list of Pauli strings.  The loop:
 //  * admits waiting requests in arrival order while their state (2^n
 //    amplitudes) and fused matrices fit in the device pools (first fit, a
-//    request that does not fit lets later, smaller ones go first);
+//    request that does not fit lets later, smaller ones go first); requests
+//    of at least heavy_qubits qubits, each of which fills the GPU alone, are
+//    admitted one at a time in arrival order, smaller ones beside them;
 //  * launches one step at a time: every admitted request advances by up to
 //    `passes_per_step` HBM passes, pass l of all of them in one launch per
 //    form, so requests admitted while others run share the launches with
@@ -66,6 +68,9 @@ public:
      // Requests of at least this many qubits (and 12) run as tile programs
      // (31 = never); applies to requests submitted afterwards.
      void set_prog_min_qubits(int q);
+     // Requests of at least q qubits are admitted one at a time (31 = no
+     // limit; default 24).
+     void set_heavy_qubits(int q);
      int64_t submit(int n, std::vector<BatchGate> gates, ServeKind kind, uint32_t shots,
                     uint64_t seed, std::vector<uint64_t> xmask, std::vector<uint64_t> zmask);
      // Collects finished steps (waiting for the oldest one if `block` and
@@ -158,6 +163,7 @@ private:
      uint64_t mat_cap_entries_ = 0;
      std::vector<int> k_by_n_;
      int prog_min_qubits_ = 12;
+     int heavy_qubits_ = 24;
      std::map<int64_t, Req> reqs_;
      std::deque<int64_t> waiting_;
      std::vector<int64_t> active_;

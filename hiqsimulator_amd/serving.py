@@ -170,12 +170,14 @@ def pauli_term(x, z):
 
 
 def make_server(precision="c64", state_bytes=32 << 30, matrix_bytes=2 << 30, passes_per_step=8,
-                max_active=1024, k_by_n=None, device=0, prog_min_qubits=12):
+                max_active=1024, k_by_n=None, device=0, prog_min_qubits=12, heavy_qubits=24):
     """A request loop with its pools; requests of at least prog_min_qubits
-    qubits run as tile programs (31 = never)."""
+    qubits run as tile programs (31 = never), requests of at least
+    heavy_qubits qubits are admitted one at a time (31 = no limit)."""
     s = Server(precision, device, int(state_bytes), int(matrix_bytes), int(passes_per_step),
                int(max_active))
     if k_by_n:
         s.set_widths(dict(k_by_n))
     s.set_prog_min_qubits(int(prog_min_qubits))
+    s.set_heavy_qubits(int(heavy_qubits))
     return s

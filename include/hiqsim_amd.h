@@ -285,6 +285,13 @@ int hiqsim_server_set_widths(hiqsim_server_t* s, const int32_t* k_by_n, size_t l
 /* Requests of at least q (and 12) qubits run as tile programs, several fused
  * ops per HBM pass (default 12; 31 = never); for later submissions. */
 int hiqsim_server_set_prog_min_qubits(hiqsim_server_t* s, int q);
+/* Requests of at least q qubits ("heavy") are admitted one at a time: a
+ * heavy request waits while another heavy one is active, smaller requests
+ * keep being admitted beside it (default 24; 31 = no limit).  A heavy state
+ * fills the GPU on its own, so running heavy requests in arrival order
+ * instead of side by side finishes each one sooner, and the next one is
+ * planned while it waits. */
+int hiqsim_server_set_heavy_qubits(hiqsim_server_t* s, int q);
 /* Gates as in hiqsim_batch_add_circuit; masks hold nterms entries (kind 1). */
 int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const double* mats,
                          const int64_t* tgt_ids, const size_t* tgt_off, const int64_t* ctrl_ids,

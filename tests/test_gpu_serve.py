@@ -142,3 +142,33 @@ def test_server_errors(engine_mod):
     assert np.array_equal(s.samples(rid), np.zeros(16, dtype=np.uint64))
     with pytest.raises(RuntimeError):
         s.expectations(rid)
+
+
+@pytest.mark.parametrize("heavy", [14, 31])
+def test_heavy_requests_one_at_a_time(engine_mod, heavy):
+    """Requests of at least heavy_qubits qubits are admitted one at a time;
+    the results do not depend on the limit."""
+    rng = np.random.default_rng(21)
+    reqs = []
+    for i in range(3):
+        gates = random_layers_circuit(14, 20, int(rng.integers(1 << 30)))
+        reqs.append(dict(arrival=0.0, n=14, gates=gates, kind="samples", shots=256, seed=50 + i,
+                         terms=[]))
+    print("serve heavy limit", heavy, flush=True)
+    # fused passes (planned at submit), so that every request can be admitted at the first step
+    s = make_server("c64", state_bytes=1 << 30, matrix_bytes=256 << 20, passes_per_step=2,
+                    heavy_qubits=heavy, prog_min_qubits=31)
+    ids = [submit(s, r) for r in reqs]
+    while not s.idle():
+        s.step(block=True)
+    got = [result(s, rid, r) for rid, r in zip(ids, reqs)]
+    peak = s.stats()["peak_active"]
+    if heavy == 14:
+        assert peak == 1
+    else:
+        assert peak == 3
+    ref = serve_all(reqs, "c64", passes_per_step=2, heavy_qubits=31, prog_min_qubits=31)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    with pytest.raises(RuntimeError):
+        s.set_heavy_qubits(0)
