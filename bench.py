@@ -376,13 +376,28 @@ def main():
         tcnt, ttot = ts.kernel_time(hq.KERNEL_CLASSES.index("tile"))
         ts.set_timing(False)
         tile_ms = ttot / tcnt if tcnt else float("nan")
+        # Algorithmic flops of the ops (8 * 2^k per amplitude per op; ops of
+        # 3+ qubits run on the matrix cores when tile_mfma is on), from the
+        # same tile plan apply_tiled makes on this fresh state.
+        pass_of, op_of, _ = hq._cppsim_amd.plan_tile_passes([g[1] for g in gates], [g[2] for g in gates], n, k)
+        support = {}
+        for gi, (p, o) in enumerate(zip(pass_of, op_of)):
+            m = 0
+            for q in list(gates[gi][1]) + list(gates[gi][2]):
+                m |= 1 << q
+            support[(p, o)] = support.get((p, o), 0) | m
+        widths = [bin(m).count("1") for m in support.values()]
+        mfma_flop = sum(8.0 * 2 ** w * 2 ** n for w in widths if w >= 3)
         tile = {"hbm_passes_per_circuit": tp, "fused_passes_per_circuit": n_pass,
                 "ms_per_circuit": round(el_t / args.steps * 1e3, 2),
                 "fused_pass_kernel_ms_per_circuit": round(dense_ms_step, 2),
                 "speedup_vs_fused_kernels": round(dense_ms_step / (el_t / args.steps * 1e3), 3),
                 "tile_pass_avg_ms": round(tile_ms, 4),
                 "tile_pass_hbm_gbs": round(32.0 * 2 ** n / (tile_ms * 1e-3) / 1e9, 1),
-                "op_width": k, "final_norm": ts.get_probability([], [])}
+                "op_width": k, "ops_per_circuit": len(widths),
+                "mfma_flop_per_circuit": mfma_flop,
+                "mfma_tflops": round(mfma_flop / (tile_ms * 1e-3 * tp) / 1e12, 2) if tp else None,
+                "final_norm": ts.get_probability([], [])}
         del ts
         log("tile passes: %s" % tile)
 
