@@ -289,8 +289,8 @@ def main():
     ap.add_argument("--tile-high-target", type=int, default=None,
                     help="dense passes with every target at or above this bit use the direct form")
     ap.add_argument("--tile-mfma", type=int, default=None,
-                    help="ops of at least 3 qubits in tile programs run on the matrix cores (1) "
-                         "or on the vector units (0)")
+                    help="tile-program op engine: 1 ops of 3+ qubits on the matrix cores, 0 all on "
+                         "the vector units")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()

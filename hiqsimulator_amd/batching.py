@@ -215,8 +215,8 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--replays", type=int, default=2)
     ap.add_argument("--tile-mfma", type=int, default=None,
-                    help="ops of at least 3 qubits in tile programs run on the matrix cores (1) "
-                         "or on the vector units (0)")
+                    help="tile-program op engine: 1 ops of 3+ qubits on the matrix cores, 0 all on "
+                         "the vector units")
     a = ap.parse_args(argv)
     from . import _cppsim_amd
     if a.tile_mfma is not None:

@@ -116,24 +116,24 @@ __device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
 // SGPR spills (a fully unrolled row loop would hold the whole matrix).  When
 // the groups outnumber the threads, each thread reads all of its groups
 // before its first write, so their LDS latencies overlap.
-template <int K, typename R, int NT, int N = kTileBits>
-__device__ __forceinline__ void wg_pass_gm(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
+template <int K, typename R, int NTH, int GN, int N = kTileBits>
+__device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5],
+                                          const uint32_t g0, const uint32_t t0)
 {
+     // thread t0 of NTH takes groups g0 + t0 + NTH * g of the GN from g0
      using V = VT<R>;
      constexpr int D = 1 << K;
-     constexpr int NG = 1 << (N - K);
-     constexpr int G = NG >= NT ? NG / NT : 1;  // groups per thread
-     static_assert(NG < NT || NG % NT == 0, "groups must divide over the threads");
+     constexpr int G = GN >= NTH ? GN / NTH : 1;  // groups per thread
+     static_assert(GN < NTH || GN % NTH == 0, "groups must divide over the threads");
      uint32_t off[D];
 #pragma unroll
      for (int j = 0; j < D; ++j) off[j] = amp_offset<K>(j, pos);
-     const uint32_t t0 = threadIdx.x;
-     if (NG < NT && t0 >= static_cast<uint32_t>(NG)) return;
+     if (GN < NTH && t0 >= static_cast<uint32_t>(GN)) return;
      uint32_t base[G];
      V v[G][D];
 #pragma unroll
      for (int g = 0; g < G; ++g) {
-          uint32_t b = t0 + static_cast<uint32_t>(g) * NT;
+          uint32_t b = g0 + t0 + static_cast<uint32_t>(g) * NTH;
 #pragma unroll
           for (int l = 0; l < K; ++l) {
                const uint32_t lo = b & ((1u << pos[l]) - 1u);
@@ -163,6 +163,12 @@ __device__ __forceinline__ void wg_pass_gm(VT<R>* psi, const VT<R>* __restrict__
                psi[base[g] + off[i]] = o;
           }
      }
+}
+
+template <int K, typename R, int NT, int N = kTileBits>
+__device__ __forceinline__ void wg_pass_gm(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
+{
+     gm_groups<K, R, NT, (1 << (N - K)), N>(psi, mat, pos, 0, threadIdx.x);
 }
 
 // The same pass on the matrix cores, for 3 <= K <= 5.  The op is a real
@@ -210,9 +216,11 @@ struct Mfma16<float> {
 };
 
 
-template <int K, typename R, int NT, int N = kTileBits>
-__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
+template <int K, typename R, int NWV, int NCB, int N = kTileBits>
+__device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5],
+                                            const int w, const uint32_t cb0)
 {
+     // wave w of NWV takes column blocks cb0 + w + NWV * i of the NCB from cb0
      static_assert(K >= 3 && K <= 5, "MFMA pass needs 3 <= K <= 5");
      using V = VT<R>;
      using M = Mfma16<R>;
@@ -220,12 +228,10 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict
      constexpr int D = 1 << K;
      constexpr int RB = D / 8;  // 16-row blocks of the real output
      constexpr int T = D / 4;   // complex input amplitudes per lane
-     constexpr int NW = NT / 64;
-     constexpr int NCB = (1 << (N - K)) / 16;  // 16-group column blocks
-     static_assert(NCB % NW == 0, "column blocks must divide over the waves");
-     constexpr int CB = NCB / NW;  // column blocks per wave
+     static_assert(NCB % NWV == 0, "column blocks must divide over the waves");
+     constexpr int CB = NCB / NWV;  // column blocks per wave
      constexpr bool kHoldA = RB * T <= 8;
-     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+     const int lane = threadIdx.x & 63;
      const int q = lane >> 4, c = lane & 15;
      // A fragments: lane (q, c) holds row c of block rb (output amplitude
      // 8*rb + row_amp(c), part row_im(c)) at input amplitudes 4t + q.
@@ -261,7 +267,7 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict
           V x[HB][T];
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
-               uint32_t b = static_cast<uint32_t>(w + (i0 + i) * NW) * 16 + c;
+               uint32_t b = (cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16 + c;
 #pragma unroll
                for (int l = 0; l < K; ++l) {
                     const uint32_t lo = b & ((1u << pos[l]) - 1u);
@@ -301,6 +307,13 @@ __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict
                }
           }
      }
+}
+
+template <int K, typename R, int NT, int N = kTileBits>
+__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
+{
+     mfma_blocks<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos,
+                                                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), 0);
 }
 
 // |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
@@ -479,9 +492,14 @@ struct MatStage {
                }
                start += sz;
           }
-          if (start == 0) return 0;  // nothing staged (no ops): mats may be empty
+          // nothing staged (no ops): mats may be empty, read its first entry never
+          const V* const src = start > 0 ? mats : nullptr;
 #pragma unroll
-          for (int i = 0; i < PER; ++i) r[i] = mats[addr[i]];
+          for (int i = 0; i < PER; ++i) {
+               V x{};
+               if (src) x = src[addr[i]];
+               r[i] = x;
+          }
           return start;
      }
      static __device__ __forceinline__ void to_lds(V* smat, const V (&r)[PER], int n)
@@ -493,6 +511,13 @@ struct MatStage {
           }
      }
 };
+
+// Op engines of the tile-program kernels: every op on the vector units, or
+// ops of 3+ qubits on the matrix cores.  (Splitting the 3-qubit ops'
+// groups between matrix-core waves and vector waves, which have the same
+// FP64 peak, measured 5-25% slower than the matrix cores alone.)
+constexpr int kTileVector = 0, kTileMatrix = 1;
+constexpr int tile_mfma_min_k(int mode) { return mode == kTileVector ? 6 : 3; }
 
 // Waves per SIMD the tile kernels are built for: two 512-thread workgroups
 // per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
@@ -509,12 +534,13 @@ constexpr int tile_waves_per_eu()
 // ops[0..nops) on the tile in LDS, a workgroup barrier before each.  The
 // first `staged` elements of the ops' matrices are in smat (back to back in
 // op order).
-template <typename R, int KMAX, int NT, int MK>
+template <typename R, int KMAX, int NT, int MODE>
 __device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const int staged,
                                          const VT<R>* __restrict__ mats,
                                          const TileOp* __restrict__ ops, const int nops)
 {
      using V = VT<R>;
+     constexpr int MK = tile_mfma_min_k(MODE);
      int soff = 0;
      for (int p = 0; p < nops; ++p) {
           const TileOp o = load_uniform(ops + p);
@@ -558,7 +584,7 @@ __device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const i
 }
 
 // One tile program on tile `tix` of the state at psi.
-template <typename R, int KMAX, int NT, int MK>
+template <typename R, int KMAX, int NT, int MODE>
 __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
                                              const TileOp* __restrict__ ops, const int nops,
                                              const uint8_t (&hpos)[kTileHigh], const uint64_t tix,
@@ -570,11 +596,12 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
      V v[IO::PER];
      IO::load(psi, tix, hpos, v);
      int staged = 0;
-     V mr[MK <= 5 ? MS::PER : 1];
-     if constexpr (MK <= 5) staged = MS::load(mr, mats, ops, nops);
+     constexpr bool kStage = MODE != kTileVector;
+     V mr[kStage ? MS::PER : 1];
+     if constexpr (kStage) staged = MS::load(mr, mats, ops, nops);
      IO::to_lds(tile, v);
-     if constexpr (MK <= 5) MS::to_lds(smat, mr, staged);
-     tile_ops<R, KMAX, NT, MK>(tile, smat, staged, mats, ops, nops);
+     if constexpr (kStage) MS::to_lds(smat, mr, staged);
+     tile_ops<R, KMAX, NT, MODE>(tile, smat, staged, mats, ops, nops);
      __syncthreads();
      IO::from_lds(tile, v);
      IO::store(psi, tix, hpos, v);
@@ -582,34 +609,34 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
 
 // Tile pass over one state: workgroup b runs the program on tile
 // swizzle(b).
-template <typename R, int KMAX, int NT, int MK>
+template <typename R, int KMAX, int NT, int MODE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
     k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
                 const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz,
                 const uint64_t nblocks)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[MK <= 5 ? tile_stage_elems<R>() : 1];
+     __shared__ VT<R> smat[MODE != kTileVector ? tile_stage_elems<R>() : 1];
      const uint64_t b = block_linear();
      if (b >= nblocks) return;  // padding of the 2-D grid (whole block)
-     tile_program<R, KMAX, NT, MK>(psi, mats, ops, nops, h.hpos, swizzle_block(b, swz), tile, smat);
+     tile_program<R, KMAX, NT, MODE>(psi, mats, ops, nops, h.hpos, swizzle_block(b, swz), tile, smat);
 }
 
 // Tile passes of many states: block b belongs to the item whose blocks
 // contain it; the item's tiles are visited in swizzled order.
-template <typename R, int KMAX, int NT, int MK>
+template <typename R, int KMAX, int NT, int MODE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
     k_batch_tprog(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
                   const BatchTileProg* __restrict__ items, const int nitems,
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[MK <= 5 ? tile_stage_elems<R>() : 1];
+     __shared__ VT<R> smat[MODE != kTileVector ? tile_stage_elems<R>() : 1];
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
      const BatchTileProg it = load_uniform(items + find_item(items, nitems, b));
      const uint64_t tix = swizzle_block(b - it.block_start, swz < it.log_blocks ? swz : it.log_blocks);
-     tile_program<R, KMAX, NT, MK>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile,
+     tile_program<R, KMAX, NT, MODE>(arena + it.state_off, mats, ops + it.op_first, it.nops, it.hpos, tix, tile,
                                    smat);
 }
 
@@ -890,14 +917,14 @@ hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const Ba
      if (kmax < 1 || kmax > 5) return hipErrorInvalidValue;
      const dim3 grid = grid_2d(total_blocks);
      const int swz = get_block_swizzle();
-     const bool mfma = get_tile_mfma() != 0;
-#define HQ_BTP(RR, KM, MKV)                                                                       \
-     hipLaunchKernelGGL((k_batch_tprog<RR, KM, 512, MKV>), grid, dim3(512), 0, s,                 \
+     const int mode = get_tile_mfma();
+#define HQ_BTP(RR, KM, MODE)                                                                       \
+     hipLaunchKernelGGL((k_batch_tprog<RR, KM, 512, MODE>), grid, dim3(512), 0, s,                 \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
                         nitems, ops, total_blocks, swz)
-#define HQ_BTP2(RR, KM)          \
-     if (mfma) HQ_BTP(RR, KM, 3); \
-     else HQ_BTP(RR, KM, 6)
+#define HQ_BTP2(RR, KM)                                  \
+     if (mode == kTileMatrix) HQ_BTP(RR, KM, kTileMatrix); \
+     else HQ_BTP(RR, KM, kTileVector)
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_BTP2(double, 5);
           else HQ_BTP2(double, 4);
@@ -923,14 +950,14 @@ hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& 
      const int bits = get_block_swizzle();
      const int swz = (bits > 0 && bits <= 16 && blocks % (1ull << bits) == 0) ? bits : 0;
      const dim3 grid = grid_2d(blocks);
-     const bool mfma = get_tile_mfma() != 0;
-#define HQ_TP(RR, KM, MKV)                                                                        \
-     hipLaunchKernelGGL((k_tile_pass<RR, KM, 512, MKV>), grid, dim3(512), 0, s,                   \
+     const int mode = get_tile_mfma();
+#define HQ_TP(RR, KM, MODE)                                                                        \
+     hipLaunchKernelGGL((k_tile_pass<RR, KM, 512, MODE>), grid, dim3(512), 0, s,                   \
                         static_cast<VT<RR>*>(psi), static_cast<const VT<RR>*>(mats), ops, nops, h, \
                         swz, blocks)
-#define HQ_TP2(RR, KM)          \
-     if (mfma) HQ_TP(RR, KM, 3); \
-     else HQ_TP(RR, KM, 6)
+#define HQ_TP2(RR, KM)                                  \
+     if (mode == kTileMatrix) HQ_TP(RR, KM, kTileMatrix); \
+     else HQ_TP(RR, KM, kTileVector)
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_TP2(double, 5);
           else HQ_TP2(double, 4);

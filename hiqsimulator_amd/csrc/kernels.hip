@@ -762,7 +762,7 @@ int get_block_swizzle() { return g_swizzle_bits; }
 void set_block_swizzle_wide(int bits) { g_swizzle_bits_wide = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
 int get_block_swizzle_wide() { return g_swizzle_bits_wide; }
 int get_block_swizzle_for(int k) { return k >= 4 ? g_swizzle_bits_wide : g_swizzle_bits; }
-void set_tile_mfma(int on) { g_tile_mfma = on ? 1 : 0; }
+void set_tile_mfma(int mode) { g_tile_mfma = mode ? 1 : 0; }
 int get_tile_mfma() { return g_tile_mfma; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,

@@ -51,9 +51,9 @@ int get_block_swizzle();
 void set_block_swizzle_wide(int bits);
 int get_block_swizzle_wide();
 int get_block_swizzle_for(int k);
-// Ops of at least 3 qubits inside tile programs run on the matrix cores (1,
-// default) or on the vector units (0).
-void set_tile_mfma(int on);
+// Op engine of tile programs: 0 every op on the vector units; 1 ops of 3+
+// qubits on the matrix cores (default).
+void set_tile_mfma(int mode);
 int get_tile_mfma();
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the

@@ -209,9 +209,9 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *                         passes of k <= 3 and the other streaming kernels
  *                         (0 = identity order, default 8, at most 16);
  *   "block_swizzle_wide": the same for dense passes of k >= 4 (default 6);
- *   "tile_mfma":          ops of at least 3 qubits inside tile programs run
- *                         on the matrix cores (1, default) or on the vector
- *                         units (0).
+ *   "tile_mfma":          op engine of tile programs: 1 (default) ops of 3+
+ *                         qubits on the matrix cores; 0 every op on the
+ *                         vector units.
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);
 int hiqsim_get_tuning(const char* name, int64_t* value);
