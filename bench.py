@@ -438,6 +438,9 @@ def main():
                      "traffic": measured_traffic(dom) if n == 30 else None,
                      "kernel": dom, "launches": cnt, "avg_launch_ms": round(per_launch_ms, 4)},
         "kernel_ms_total": kernel_share,
+        "rates": {"fused_passes_per_s": round(n_pass / (ms_step * 1e-3), 1),
+                  "gates_per_s": round(len(gates) / (ms_step * 1e-3), 1),
+                  "permutations_per_s": round(n_perm / (ms_step * 1e-3), 2)},
         "k_sweep_gbs": sweep,
         "tile_planner": tile,
         "cpu_baseline": cpu,
