@@ -461,7 +461,7 @@ struct TileIo {
 template <typename R>
 constexpr int tile_stage_elems()
 {
-     return sizeof(R) == 8 ? 1024 : 2048;  // 16 KiB
+     return 1024;  // 16 KiB complex128, 8 KiB complex64
 }
 
 template <typename R, int NT>
@@ -521,14 +521,14 @@ constexpr int tile_mfma_min_k(int mode) { return mode == kTileVector ? 6 : 3; }
 
 // Waves per SIMD the tile kernels are built for: two 512-thread workgroups
 // per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
-// VGPRs), three for complex64 (48 KiB; at most 80 VGPRs).  Programs with
+// VGPRs), four for complex64 (40 KiB; at most 64 VGPRs).  Programs with
 // 5-qubit ops get more registers (complex128 unconstrained, complex64 two
 // workgroups per CU): their 5-qubit passes need them.
 template <typename R, int KMAX>
 constexpr int tile_waves_per_eu()
 {
      if (sizeof(R) == 8) return KMAX >= 5 ? 1 : 4;
-     return KMAX >= 5 ? 4 : 6;
+     return KMAX >= 5 ? 4 : 8;
 }
 
 // ops[0..nops) on the tile in LDS, a workgroup barrier before each.  The
