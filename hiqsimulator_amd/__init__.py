@@ -31,5 +31,34 @@ Server = _cppsim_amd.Server
 KERNEL_CLASSES = ("dense_k1", "dense_k2", "dense_k3", "dense_k4", "dense_k5", "diag", "scale",
                   "permute", "reduce", "normalize", "remove_bit", "fill_copy", "pauli", "tile")
 
+
+
+def _hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return {line.split()[-1] for line in f if "libamdhip64" in line}
+    except OSError:
+        return set()
+
+
+def state_view(sim):
+    """``sim``'s state as a torch tensor on its GPU, without a copy
+    (DLPack): 2^n complex128 or complex64 amplitudes, index bit p = the
+    qubit ``sim.get_qubits_ids()[p]``.  Pending gates are run first.  The
+    tensor aliases the simulator's buffer until the qubit count changes;
+    writes through it change the state (synchronize torch's stream before
+    the next call on ``sim``).
+
+    torch wheels carry their own HIP runtime.  The two libraries share one
+    runtime (and so can share device memory) only when torch is imported
+    before ``hiqsimulator_amd``; otherwise this raises RuntimeError."""
+    import torch
+    if len(_hip_runtimes()) > 1:
+        raise RuntimeError("state_view(): torch and hiqsimulator_amd loaded different HIP runtimes; "
+                           "import torch before hiqsimulator_amd to share one")
+    return torch.from_dlpack(sim.state_dlpack())
+
+
 __all__ = ["SimulatorMPI", "SwapScheduler", "ClusterScheduler", "Batch", "Server", "device_count",
-           "KERNEL_CLASSES", "LIB_PATH"]
+           "KERNEL_CLASSES", "LIB_PATH", "state_view"]

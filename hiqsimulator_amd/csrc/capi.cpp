@@ -321,6 +321,37 @@ int hiqsim_state_size(hiqsim_t* h, uint64_t* out)
      });
 }
 
+int hiqsim_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precision, int* device)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(ptr, "ptr");
+          need(n_amps, "n_amps");
+          *ptr = h->eng.device_state();
+          *n_amps = h->eng.state_size();
+          if (precision) *precision = static_cast<int>(h->eng.precision());
+          if (device) *device = h->eng.device();
+     });
+}
+
+int hiqsim_save_state(hiqsim_t* h, const char* path)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(path, "path");
+          h->eng.save_state(path);
+     });
+}
+
+int hiqsim_load_state(hiqsim_t* h, const char* path)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(path, "path");
+          h->eng.load_state(path);
+     });
+}
+
 int hiqsim_get_state(hiqsim_t* h, double* out, uint64_t n_amps)
 {
      return guard([&] {

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <sstream>
@@ -657,6 +658,121 @@ double Engine::entropy()
 // ---------------------------------------------------------------------------
 // State transfer
 // ---------------------------------------------------------------------------
+void* Engine::device_state()
+{
+     DeviceScope ds(device_);
+     run();
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     return d_state_;
+}
+
+namespace {
+constexpr char kStateMagic[4] = {'H', 'Q', 'S', 'V'};
+constexpr uint32_t kStateVersion = 1;
+constexpr size_t kStageBytes = size_t(64) << 20;
+
+struct File {
+     FILE* f = nullptr;
+     File(const std::string& path, const char* mode) : f(std::fopen(path.c_str(), mode))
+     {
+          if (!f) throw std::runtime_error("state file '" + path + "': cannot open");
+     }
+     ~File()
+     {
+          if (f) std::fclose(f);
+     }
+     void read(void* p, size_t n)
+     {
+          if (std::fread(p, 1, n, f) != n) throw std::runtime_error("state file: truncated");
+     }
+     void write(const void* p, size_t n)
+     {
+          if (std::fwrite(p, 1, n, f) != n) throw std::runtime_error("state file: write failed");
+     }
+};
+
+struct Pinned {
+     void* p = nullptr;
+     explicit Pinned(size_t n) { check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc(stage)"); }
+     ~Pinned()
+     {
+          if (p) (void)hipHostFree(p);
+     }
+};
+}  // namespace
+
+void Engine::save_state(const std::string& path)
+{
+     DeviceScope ds(device_);
+     run();
+     File f(path, "wb");
+     const uint32_t hdr[3] = {kStateVersion, static_cast<uint32_t>(prec_), static_cast<uint32_t>(locals_.size())};
+     f.write(kStateMagic, 4);
+     f.write(hdr, sizeof(hdr));
+     f.write(locals_.data(), locals_.size() * sizeof(int64_t));
+     const size_t total = n_amps_ * amp_bytes(prec_);
+     Pinned stage(std::min(total, kStageBytes));
+     for (size_t off = 0; off < total; off += kStageBytes) {
+          const size_t n = std::min(kStageBytes, total - off);
+          check(hipMemcpyAsync(stage.p, static_cast<const char*>(d_state_) + off, n, hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync(save)");
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          f.write(stage.p, n);
+     }
+}
+
+void Engine::load_state(const std::string& path)
+{
+     DeviceScope ds(device_);
+     run();
+     File f(path, "rb");
+     char magic[4];
+     uint32_t hdr[3];
+     f.read(magic, 4);
+     f.read(hdr, sizeof(hdr));
+     if (std::memcmp(magic, kStateMagic, 4) != 0 || hdr[0] != kStateVersion)
+          throw std::runtime_error("state file '" + path + "': not a saved state");
+     if (hdr[1] > 1) throw std::runtime_error("state file '" + path + "': unknown precision");
+     const Prec fprec = static_cast<Prec>(hdr[1]);
+     const uint32_t n = hdr[2];
+     if (n != locals_.size())
+          throw std::runtime_error("load_state(): the file holds " + std::to_string(n) + " qubits, the simulator " +
+                                   std::to_string(locals_.size()));
+     std::vector<int64_t> ids(n);
+     f.read(ids.data(), n * sizeof(int64_t));
+     {
+          std::vector<int64_t> a = ids, b = locals_;
+          std::sort(a.begin(), a.end());
+          std::sort(b.begin(), b.end());
+          if (a != b) throw std::runtime_error("load_state(): the file's qubit ids differ from the allocated ones");
+     }
+     set_qubits_perm(ids);
+     const size_t fab = amp_bytes(fprec), ab = amp_bytes(prec_);
+     const uint64_t chunk = kStageBytes / std::max(fab, ab);
+     Pinned stage(std::min<size_t>(n_amps_, chunk) * ab);
+     std::vector<char> conv(fprec == prec_ ? 0 : std::min<size_t>(n_amps_, chunk) * fab);
+     for (uint64_t a0 = 0; a0 < n_amps_; a0 += chunk) {
+          const uint64_t na = std::min<uint64_t>(chunk, n_amps_ - a0);
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");  // stage free again
+          if (fprec == prec_) {
+               f.read(stage.p, na * ab);
+          }
+          else {
+               f.read(conv.data(), na * fab);
+               for (uint64_t i = 0; i < 2 * na; ++i) {
+                    if (prec_ == Prec::C128)
+                         static_cast<double*>(stage.p)[i] = reinterpret_cast<const float*>(conv.data())[i];
+                    else
+                         static_cast<float*>(stage.p)[i] =
+                             static_cast<float>(reinterpret_cast<const double*>(conv.data())[i]);
+               }
+          }
+          check(hipMemcpyAsync(static_cast<char*>(d_state_) + a0 * ab, stage.p, na * ab, hipMemcpyHostToDevice, stream_),
+                "hipMemcpyAsync(load)");
+     }
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
 void Engine::get_state_c128(std::complex<double>* out)
 {
      DeviceScope ds(device_);

@@ -82,7 +82,18 @@ public:
      uint64_t state_size() const { return n_amps_; }
      Prec precision() const { return prec_; }
      void get_state_c128(std::complex<double>* out);
+     // The state's device buffer after pending gates have run and the
+     // engine's stream has drained; valid until the qubit count changes.
+     void* device_state();
      void set_state_c128(const std::complex<double>* in, uint64_t n);
+     // Save/restore of the interleaved state to a file, streamed through a
+     // pinned staging buffer (no host copy of the whole state).  Format:
+     // "HQSV", u32 version 1, u32 precision (0 complex128, 1 complex64),
+     // u32 n, n x i64 qubit ids (bit order), 2^n interleaved amplitudes.
+     // load_state needs the same set of qubit ids; it adopts the file's bit
+     // order (a relabel, as set_qubits_perm) and converts the precision.
+     void save_state(const std::string& path);
+     void load_state(const std::string& path);
 
      // Device-side checkpoint: snapshot() copies the state (and the qubit
      // order) into a second HBM buffer with one streaming copy; restore()

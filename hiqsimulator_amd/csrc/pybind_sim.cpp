@@ -22,6 +22,52 @@ namespace py = pybind11;
 
 namespace {
 
+// DLPack exchange structs (dlpack.h, the published DLPack ABI that
+// torch.from_dlpack and numpy read), declared here from the specification.
+struct DLDevice {
+     int32_t device_type;
+     int32_t device_id;
+};
+struct DLDataType {
+     uint8_t code;
+     uint8_t bits;
+     uint16_t lanes;
+};
+struct DLTensor {
+     void* data;
+     DLDevice device;
+     int32_t ndim;
+     DLDataType dtype;
+     int64_t* shape;
+     int64_t* strides;
+     uint64_t byte_offset;
+};
+struct DLManagedTensor {
+     DLTensor dl_tensor;
+     void* manager_ctx;
+     void (*deleter)(DLManagedTensor*);
+};
+constexpr int32_t kDLROCM = 10;
+constexpr uint8_t kDLComplex = 5;
+
+// A DLPack view of a simulator's device state; it keeps the Python
+// simulator object alive until the consumer releases the view.
+struct StateView {
+     DLManagedTensor t{};
+     int64_t shape[1] = {0};
+     PyObject* owner = nullptr;
+};
+
+void state_view_deleter(DLManagedTensor* t)
+{
+     auto* v = static_cast<StateView*>(t->manager_ctx);
+     {
+          py::gil_scoped_acquire g;
+          Py_XDECREF(v->owner);
+     }
+     delete v;
+}
+
 using cvec = std::vector<std::complex<double>>;
 
 void ok(int rc)
@@ -224,6 +270,50 @@ public:
                ok(hiqsim_get_state(h_, p, n));
           }
           return a;
+     }
+     // The state without a copy, as a DLPack capsule ("dltensor") for
+     // torch.from_dlpack: 2^n complex128/complex64 amplitudes on the GPU,
+     // valid until the qubit count changes.
+     static py::capsule state_dlpack(py::object self_obj)
+     {
+          PySim& self = self_obj.cast<PySim&>();
+          void* ptr = nullptr;
+          uint64_t n = 0;
+          int prec = 0, dev = 0;
+          {
+               py::gil_scoped_release r;
+               ok(hiqsim_device_state(self.h_, &ptr, &n, &prec, &dev));
+          }
+          auto* v = new StateView();
+          v->shape[0] = static_cast<int64_t>(n);
+          v->t.dl_tensor.data = ptr;
+          v->t.dl_tensor.device = DLDevice{kDLROCM, dev};
+          v->t.dl_tensor.ndim = 1;
+          v->t.dl_tensor.dtype = DLDataType{kDLComplex, static_cast<uint8_t>(prec == HIQSIM_C128 ? 128 : 64), 1};
+          v->t.dl_tensor.shape = v->shape;
+          v->t.dl_tensor.strides = nullptr;
+          v->t.dl_tensor.byte_offset = 0;
+          v->t.manager_ctx = v;
+          v->t.deleter = state_view_deleter;
+          v->owner = self_obj.ptr();
+          Py_INCREF(v->owner);
+          return py::capsule(&v->t, "dltensor", [](PyObject* cap) {
+               // an unconsumed capsule still owns the view
+               if (PyCapsule_IsValid(cap, "dltensor")) {
+                    auto* t = static_cast<DLManagedTensor*>(PyCapsule_GetPointer(cap, "dltensor"));
+                    t->deleter(t);
+               }
+          });
+     }
+     void save_state(const std::string& path)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_save_state(h_, path.c_str()));
+     }
+     void load_state(const std::string& path)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_load_state(h_, path.c_str()));
      }
      // SimulatorMPI::cheat_local (SimulatorMPI.cpp:543-559): id -> bit
      // position and a copy of the state.
@@ -709,6 +799,9 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("cheat_local", &PySim::cheat_local)
          .def("collapse_wavefunction", &PySim::collapse_wavefunction)
          .def("get_state", &PySim::get_state)
+         .def("state_dlpack", &PySim::state_dlpack)
+         .def("save_state", &PySim::save_state)
+         .def("load_state", &PySim::load_state)
          .def("set_state", &PySim::set_state)
          .def("set_wavefunction", &PySim::set_wavefunction)
          .def("get_expectation_value", &PySim::get_expectation_value)

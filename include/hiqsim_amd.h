@@ -100,6 +100,24 @@ int hiqsim_state_size(hiqsim_t* h, uint64_t* out);
 /* cheat_local (_cppsim_mpi.cpp:81, SimulatorMPI.cpp:543-559): copy the state
  * out as interleaved complex128 (n_amps must equal hiqsim_state_size). */
 int hiqsim_get_state(hiqsim_t* h, double* out, uint64_t n_amps);
+/* Zero-copy access to the state (the reference only copies it out,
+ * cheat_local, SimulatorMPI.cpp:543-559): runs pending gates, drains the
+ * handle's stream and returns the device buffer of n_amps interleaved
+ * amplitudes (complex128 or complex64 per *precision, HIQSIM_C128 /
+ * HIQSIM_C64) on HIP device *device.  The pointer stays valid until the
+ * next allocation or deallocation of a qubit or the handle's destruction;
+ * work the caller queues on other streams must be complete before the next
+ * call on the handle.  precision and device may be NULL. */
+int hiqsim_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precision, int* device);
+/* Save/restore of the state to a file (no reference counterpart; the
+ * reference can only copy the state out, cheat_local), streamed through a
+ * 64 MiB pinned buffer.  Format: "HQSV", u32 version 1, u32 precision
+ * (HIQSIM_C128 / HIQSIM_C64), u32 n, n x int64 qubit ids in bit order, then
+ * 2^n interleaved amplitudes in that precision, all little-endian.
+ * load_state needs the handle to hold the same set of qubit ids; it adopts
+ * the file's bit order (a relabel) and converts the precision. */
+int hiqsim_save_state(hiqsim_t* h, const char* path);
+int hiqsim_load_state(hiqsim_t* h, const char* path);
 /* set_wavefunction (_simulator_mpi.py:281-306; declared but unbound in the
  * reference): replace the amplitudes, qubit order unchanged. */
 int hiqsim_set_state(hiqsim_t* h, const double* in, uint64_t n_amps);
