@@ -433,6 +433,19 @@ int hiqsim_apply_pauli_sum(hiqsim_t* h, const uint64_t* xmask, const uint64_t* z
      });
 }
 
+int hiqsim_emulate_time_evolution(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
+                                  const double* coeff, size_t nterms, double time, const int64_t* ids,
+                                  size_t nids, const int64_t* ctrls, size_t nctrls)
+{
+     return guard([&] {
+          need(h, "handle");
+          std::vector<uint64_t> X, Z;
+          std::vector<hq::cplx> C;
+          pauli_args(xmask, zmask, coeff, nterms, X, Z, C);
+          h->eng.time_evolution(X, Z, C, time, ivec(ids, nids), ivec(ctrls, nctrls));
+     });
+}
+
 int hiqsim_apply_matrix(hiqsim_t* h, const double* m, int k, const uint32_t* pos,
                         uint64_t ctrlmask, int diag)
 {

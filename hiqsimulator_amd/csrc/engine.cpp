@@ -1150,4 +1150,99 @@ void Engine::apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<
      swap_tmp();
 }
 
+void Engine::time_evolution(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                            const std::vector<cplx>& coeff, double t, const std::vector<int64_t>& ids,
+                            const std::vector<int64_t>& ctrls)
+{
+     DeviceScope ds(device_);
+     if (xm.size() != zm.size() || xm.size() != coeff.size())
+          throw std::runtime_error("emulate_time_evolution(): term arrays differ in length");
+     if (!std::isfinite(t)) throw std::runtime_error("emulate_time_evolution(): time must be finite");
+     run();
+     std::vector<uint32_t> pos(ids.size());
+     for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
+     uint64_t cmask = 0, tmask = 0;
+     for (auto p: pos) tmask |= uint64_t(1) << p;
+     for (auto c: ctrls) cmask |= uint64_t(1) << find_sure(c);
+     if (cmask & tmask) throw std::runtime_error("emulate_time_evolution(): a control is also a target");
+     auto to_bits = [&](uint64_t m) {
+          uint64_t r = 0;
+          for (size_t i = 0; i < 64; ++i)
+               if ((m >> i) & 1) {
+                    if (i >= ids.size())
+                         throw std::runtime_error("qubit_operator acts on more qubits than contained in the qureg.");
+                    r |= uint64_t(1) << pos[i];
+               }
+          return r;
+     };
+     // identity terms -> phase; the others with i^{|x & z|} folded in (Y = i X Z)
+     cplx c0(0, 0);
+     double l1 = 0;
+     std::vector<uint64_t> X, Z;
+     std::vector<cplx> C;
+     for (size_t k = 0; k < xm.size(); ++k) {
+          if (xm[k] == 0 && zm[k] == 0) {
+               c0 += coeff[k];
+               continue;
+          }
+          X.push_back(to_bits(xm[k]));
+          Z.push_back(to_bits(zm[k]));
+          C.push_back(coeff[k] * ipow(__builtin_popcountll(xm[k] & zm[k])));
+          l1 += std::abs(coeff[k]);
+     }
+     const size_t bytes = n_amps_ * amp_bytes(prec_);
+     struct Buf {
+          void* p = nullptr;
+          explicit Buf(size_t n) { check(hipMalloc(&p, n), "hipMalloc(time evolution)"); }
+          ~Buf()
+          {
+               if (p) (void)hipFree(p);
+          }
+     };
+     // w: the controlled part of the state, evolved step by step
+     Buf w(bytes), out(bytes), a(bytes), b(bytes);
+     check(launch_copy(w.p, d_state_, prec_, n_amps_, stream_), "copy");
+     if (cmask) {
+          check(launch_normalize(w.p, prec_, n_amps_, cmask, cmask, 1.0, stream_), "select");
+          // the state keeps only its uncontrolled part until w comes back
+          check(launch_pauli_accumulate(d_state_, w.p, prec_, n_amps_, 0, 0, -1.0, 0.0, stream_), "subtract");
+     }
+     const uint64_t steps = X.empty() ? 0 : std::max<uint64_t>(1, static_cast<uint64_t>(std::ceil(std::abs(t) * l1)));
+     const double tau = steps ? t / static_cast<double>(steps) : 0.0;
+     constexpr int kMaxTerms = 64;
+     void* pw = w.p;
+     void* po = out.p;
+     for (uint64_t s = 0; s < steps; ++s) {
+          check(launch_copy(po, pw, prec_, n_amps_, stream_), "copy");
+          void* pa = a.p;
+          void* pb = b.p;
+          check(launch_copy(pa, pw, prec_, n_amps_, stream_), "copy");
+          for (int k = 1; k <= kMaxTerms; ++k) {
+               // b = (-i tau / k) H a; out += b
+               const cplx f = cplx(0, -tau / k);
+               check(launch_fill(pb, prec_, n_amps_, 0.0, 0.0, stream_), "fill");
+               for (size_t j = 0; j < X.size(); ++j) {
+                    const cplx c = f * C[j];
+                    check(launch_pauli_accumulate(pb, pa, prec_, n_amps_, X[j], Z[j], c.real(), c.imag(), stream_),
+                          "pauli_accumulate");
+               }
+               check(launch_pauli_accumulate(po, pb, prec_, n_amps_, 0, 0, 1.0, 0.0, stream_), "accumulate");
+               check(launch_masked_norm(pb, prec_, n_qubits(), 0, 0, d_scratch_, d_scratch_ + kReduceScratchDoubles,
+                                        stream_),
+                     "norm");
+               read_out(h_out_, 1);
+               std::swap(pa, pb);
+               if (h_out_[0] < 1e-24) break;
+          }
+          std::swap(pw, po);
+     }
+     if (c0 != cplx(0, 0)) {
+          const cplx ph = std::exp(cplx(0, -t) * c0);
+          check(launch_scale(pw, prec_, n_amps_, ph.real(), ph.imag(), stream_), "phase");
+     }
+     if (cmask) check(launch_pauli_accumulate(d_state_, pw, prec_, n_amps_, 0, 0, 1.0, 0.0, stream_), "add");
+     else check(launch_copy(d_state_, pw, prec_, n_amps_, stream_), "copy");
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
 }  // namespace hq

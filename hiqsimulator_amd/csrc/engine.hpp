@@ -117,6 +117,15 @@ public:
                       const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
      void apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
                           const std::vector<cplx>& coeff, const std::vector<int64_t>& ids);
+     // exp(-i t H) on the subspace where every control qubit is 1, H the
+     // Pauli sum (masks over ids, as above).  The reference's engine calls
+     // this for ProjectQ's TimeEvolution gate (_simulator_mpi.py:469-475)
+     // but its C++ never defines it.  Identity terms become a phase; the
+     // rest is a Taylor series in s = ceil(|t| sum|c|) steps, each summed
+     // until a term's norm drops below 1e-12 (at most 64 terms).
+     void time_evolution(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
+                         const std::vector<cplx>& coeff, double t, const std::vector<int64_t>& ids,
+                         const std::vector<int64_t>& ctrls);
 
      // Applies a gate list (ids as in apply_gate; matrices 2^t x 2^t with t
      // targets) with tile passes: each HBM pass loads tiles of index bits

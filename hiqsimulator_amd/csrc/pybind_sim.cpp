@@ -394,6 +394,18 @@ public:
           ok(hiqsim_apply_pauli_sum(h_, X.data(), Z.data(), C.data(), X.size(), ids.data(),
                                     ids.size()));
      }
+     // _simulator_mpi.py:469-475: the call the reference engine makes for a
+     // TimeEvolution gate (terms as for apply_qubit_operator).
+     void emulate_time_evolution(const py::list& terms, double time, const std::vector<int64_t>& ids,
+                                 const std::vector<int64_t>& ctrls)
+     {
+          std::vector<uint64_t> X, Z;
+          std::vector<double> C;
+          parse_terms(terms, X, Z, C);
+          py::gil_scoped_release r;
+          ok(hiqsim_emulate_time_evolution(h_, X.data(), Z.data(), C.data(), X.size(), time, ids.data(),
+                                           ids.size(), ctrls.data(), ctrls.size()));
+     }
      void apply_matrix(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> m,
                        const std::vector<uint32_t>& pos, uint64_t ctrlmask, bool diag)
      {
@@ -807,6 +819,7 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("get_expectation_value", &PySim::get_expectation_value)
          .def("expectation_complex", &PySim::expectation_complex)
          .def("apply_qubit_operator", &PySim::apply_qubit_operator)
+         .def("emulate_time_evolution", &PySim::emulate_time_evolution)
          .def("apply_matrix", &PySim::apply_matrix, py::arg("matrix"), py::arg("pos"),
               py::arg("ctrlmask") = 0, py::arg("diag") = false)
          .def("snapshot", &PySim::snapshot)

@@ -150,7 +150,12 @@ class SimulatorMPI(BasicEngine):
                                          [[qb.id for qb in qr] for qr in cmd.qubits],
                                          [qb.id for qb in cmd.control_qubits])
         elif isinstance(gate, TimeEvolution):
-            raise Exception("SimulatorMPI does not support TimeEvolution (as the reference).")
+            # _simulator_mpi.py:469-475 (is_available still declines the
+            # gate, as the reference does, so ProjectQ decomposes it unless
+            # a caller sends it here directly)
+            op = [(list(term), coeff) for (term, coeff) in gate.hamiltonian.terms.items()]
+            self._simulator.emulate_time_evolution(op, gate.time, [qb.id for qb in cmd.qubits[0]],
+                                                   [qb.id for qb in cmd.control_qubits])
         elif len(gate.matrix) <= 2 ** 5:
             ids = [qb.id for qr in cmd.qubits for qb in qr]
             matrix = numpy.asarray(gate.matrix, dtype=complex)

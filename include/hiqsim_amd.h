@@ -153,6 +153,15 @@ int hiqsim_expectation_pauli(hiqsim_t* h, const uint64_t* xmask, const uint64_t*
  * psi <- sum_t coeff[t] P_t psi (not renormalised). */
 int hiqsim_apply_pauli_sum(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
                            const double* coeff, size_t nterms, const int64_t* ids, size_t nids);
+/* emulate_time_evolution (called by _simulator_mpi.py:469-475 for ProjectQ's
+ * TimeEvolution gate, never defined in the reference's C++): psi <- exp(-i
+ * time H) psi on the subspace where every ctrls qubit is 1, H the Pauli sum
+ * as above (Hermitian for a unitary result).  Identity terms become a phase;
+ * the rest is a Taylor series in ceil(|time| sum|coeff|) steps, each summed
+ * until a term's norm is below 1e-12. */
+int hiqsim_emulate_time_evolution(hiqsim_t* h, const uint64_t* xmask, const uint64_t* zmask,
+                                  const double* coeff, size_t nterms, double time, const int64_t* ids,
+                                  size_t nids, const int64_t* ctrls, size_t nctrls);
 
 /* One immediate fused pass at explicit bit positions, bypassing fusion: the
  * reference's kernelK / kernel_core(_diag) call in Run()
