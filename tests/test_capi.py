@@ -44,7 +44,8 @@ def test_pybind_surface_matches_reference():
                    "get_amplitude", "get_probability", "run", "entropy", "cheat_local",
                    "collapse_wavefunction"]
     extra = ["get_expectation_value", "apply_qubit_operator", "set_wavefunction",
-             "permute_qubits", "get_state", "set_state", "state_dlpack", "save_state", "load_state"]
+             "permute_qubits", "get_state", "set_state", "state_dlpack", "save_state", "load_state",
+             "emulate_time_evolution"]
     for name in ref_methods + extra:
         assert hasattr(hiqsimulator_amd.SimulatorMPI, name), name
 
