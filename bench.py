@@ -176,6 +176,34 @@ def cpu_baseline(k, seconds=12.0, n_sample=28):
                       % (passes, k, n_sample, el)}
 
 
+def bench_precision(hq, n, log_fn):
+    """The QFT-like circuit (H on each qubit, controlled phases 2*pi/2^k on
+    every pair) and its inverse on a random normalised complex Gaussian
+    state, in complex128 and complex64; the round trip is compared with the
+    initial state on the device in FP64 (max-abs, fidelity, norm drift)."""
+    from hiqsimulator_amd.circuits import qft_like_circuit
+    from hiqsimulator_amd.scheduling import apply_scheduled
+    gates = qft_like_circuit(n)
+    inv = [(m.conj().T, t, c) for (m, t, c) in reversed(gates)]
+    out = {"workload": "%dq QFT-like circuit (%d gates) and its inverse on a random state" % (n, len(gates))}
+    for prec in ("c128", "c64"):
+        g = hq.SimulatorMPI(1, n, 4, prec)
+        g.allocate_qureg(list(range(n)), 0)
+        g.init_random_state(n)
+        g.snapshot()
+        t0 = time.perf_counter()
+        apply_scheduled(g, gates, 4)
+        apply_scheduled(g, inv, 4)
+        g.synchronize()
+        el = time.perf_counter() - t0
+        r = g.compare_snapshot()
+        out[prec] = {"max_abs": r["max_abs_diff"], "one_minus_fidelity": max(0.0, 1.0 - r["fidelity"]),
+                     "norm_drift": abs(r["state_norm2"] - r["ref_norm2"]), "seconds": round(el, 2)}
+        del g
+        log_fn("precision %s: %s" % (prec, out[prec]))
+    return out
+
+
 def bench_batch(hq, count, seed, log_fn, plain_sample=64):
     """Many-circuit batch: ``count`` circuits, n ~ U{10..22}, depth ~ U{10..60},
     a Haar 2-qubit unitary every 5th layer, all in one graph launch, complex64
@@ -277,6 +305,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-tile", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-qubits", type=int, default=32)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
@@ -407,6 +437,7 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(k)
     if cpu:
         log("cpu baseline: %s" % cpu)
+    precision = None if args.no_precision else bench_precision(hq, args.precision_qubits, log)
     batch = None if args.no_batch else bench_batch(hq, args.batch_circuits, args.seed, log)
     serving = None
     if not args.no_serving:
@@ -444,6 +475,7 @@ def main():
         "k_sweep_gbs": sweep,
         "tile_planner": tile,
         "cpu_baseline": cpu,
+        "precision": precision,
         "batch": batch,
         "serving": serving,
     }
