@@ -746,8 +746,19 @@ void Engine::load_state(const std::string& path)
           std::sort(b.begin(), b.end());
           if (a != b) throw std::runtime_error("load_state(): the file's qubit ids differ from the allocated ones");
      }
-     set_qubits_perm(ids);
      const size_t fab = amp_bytes(fprec), ab = amp_bytes(prec_);
+     {
+          // the whole payload must be there before the state is touched
+          const long here = std::ftell(f.f);
+          if (here < 0 || std::fseek(f.f, 0, SEEK_END) != 0)
+               throw std::runtime_error("state file '" + path + "': cannot seek");
+          const long end = std::ftell(f.f);
+          if (end < 0 || static_cast<uint64_t>(end - here) != n_amps_ * fab)
+               throw std::runtime_error("state file '" + path + "': expected " + std::to_string(n_amps_ * fab) +
+                                        " bytes of amplitudes");
+          if (std::fseek(f.f, here, SEEK_SET) != 0) throw std::runtime_error("state file '" + path + "': cannot seek");
+     }
+     set_qubits_perm(ids);
      const uint64_t chunk = kStageBytes / std::max(fab, ab);
      Pinned stage(std::min<size_t>(n_amps_, chunk) * ab);
      std::vector<char> conv(fprec == prec_ ? 0 : std::min<size_t>(n_amps_, chunk) * fab);

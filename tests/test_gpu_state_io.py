@@ -124,6 +124,12 @@ def test_load_errors(engine_mod, tmp_path):
     c.allocate_qureg(list(range(10, 16)), 0)
     with pytest.raises(RuntimeError):
         c.load_state(path)  # other qubit ids
+    truncated = tmp_path / "short.hqsv"
+    truncated.write_bytes(open(path, "rb").read()[:-16])
+    before = a.get_state()
+    with pytest.raises(RuntimeError):
+        a.load_state(str(truncated))
+    assert np.array_equal(a.get_state(), before)  # untouched
     bad = tmp_path / "bad.hqsv"
     bad.write_bytes(b"not a state")
     with pytest.raises(RuntimeError):
