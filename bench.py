@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--no-tile", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--precision-qubits", type=int, default=32)
+    ap.add_argument("--tile-op-width", type=int, default=None,
+                    help="op width of the tile programs (default: --k)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
@@ -391,16 +393,17 @@ def main():
     # kernel time of the run above.
     tile = None
     if not args.no_tile:
+        tw = args.tile_op_width or k
         ts = hq.SimulatorMPI(args.seed, n, k)
         ts.allocate_qureg(list(range(n)), 0)
         for w in range(args.warmup):
-            ts.apply_tiled(gates, k)
+            ts.apply_tiled(gates, tw)
         ts.synchronize()
         ts.set_timing(True)
         t0 = time.perf_counter()
         tp = 0
         for s in range(args.steps):
-            tp = ts.apply_tiled(gates, k)
+            tp = ts.apply_tiled(gates, tw)
         ts.synchronize()
         el_t = time.perf_counter() - t0
         tcnt, ttot = ts.kernel_time(hq.KERNEL_CLASSES.index("tile"))
@@ -409,7 +412,7 @@ def main():
         # Algorithmic flops of the ops (8 * 2^k per amplitude per op; ops of
         # 3+ qubits run on the matrix cores when tile_mfma is on), from the
         # same tile plan apply_tiled makes on this fresh state.
-        pass_of, op_of, _ = hq._cppsim_amd.plan_tile_passes([g[1] for g in gates], [g[2] for g in gates], n, k)
+        pass_of, op_of, _ = hq._cppsim_amd.plan_tile_passes([g[1] for g in gates], [g[2] for g in gates], n, tw)
         support = {}
         for gi, (p, o) in enumerate(zip(pass_of, op_of)):
             m = 0
@@ -424,7 +427,7 @@ def main():
                 "speedup_vs_fused_kernels": round(dense_ms_step / (el_t / args.steps * 1e3), 3),
                 "tile_pass_avg_ms": round(tile_ms, 4),
                 "tile_pass_hbm_gbs": round(32.0 * 2 ** n / (tile_ms * 1e-3) / 1e9, 1),
-                "op_width": k, "ops_per_circuit": len(widths),
+                "op_width": tw, "ops_per_circuit": len(widths),
                 "mfma_flop_per_circuit": mfma_flop,
                 "mfma_tflops": round(mfma_flop / (tile_ms * 1e-3 * tp) / 1e12, 2) if tp else None,
                 "final_norm": ts.get_probability([], [])}
