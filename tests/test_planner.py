@@ -135,3 +135,98 @@ def test_tile_plan_errors():
         _cppsim_amd.plan_tile_passes([[0]], [[]], 11, 4)  # fewer than 12 bits
     with pytest.raises(RuntimeError):
         _cppsim_amd.plan_tile_passes([[12]], [[]], 12, 4)  # bit out of range
+
+
+# ---- brute-force pins of the scheduler objectives ---------------------------
+def _random_gate_list(rng, nq, ngates):
+    gates, ctrls, diag = [], [], []
+    for _ in range(ngates):
+        nt = int(rng.integers(1, 3))
+        nc = int(rng.integers(0, 2))
+        q = [int(x) for x in rng.choice(nq, size=nt + nc, replace=False)]
+        gates.append(q[:nt])
+        ctrls.append(q[nt:])
+        diag.append(bool(rng.random() < 0.3))
+    return gates, ctrls, diag
+
+
+def _cluster_taken(gates, ctrls, diag, local, cluster):
+    """Gates a ClusterScheduler cluster takes (cluster_scheduler.h:146-160):
+    in order, a gate is taken if it touches no qubit of a skipped gate, its
+    local qubits lie in the cluster and, unless diagonal, its targets are
+    all local; a skipped gate's qubits block every later gate on them."""
+    bad, taken = set(), []
+    for i, (t, c) in enumerate(zip(gates, ctrls)):
+        q = set(t) | set(c)
+        if not (q & bad) and (q & local) <= cluster and (diag[i] or set(t) <= local):
+            taken.append(i)
+        else:
+            bad |= q
+    return taken
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_cluster_scheduler_objective_brute_force(seed):
+    """ScheduleCluster returns the gates of a cluster (a set of at most
+    cluster_size local qubits) that takes the most gates, and among those
+    the fewest qubits (cluster_scheduler.h:76-92, .cpp:40-77); with no such
+    cluster, the first gate takeable on all local qubits.  Every cluster is
+    enumerated here; only the choice among equal clusters is the search
+    order's."""
+    import itertools
+    import hiqsimulator_amd as hq
+    rng = np.random.default_rng(seed)
+    nq = 7
+    gates, ctrls, diag = _random_gate_list(rng, nq, 12)
+    local = set(int(x) for x in rng.choice(nq, size=int(rng.integers(4, 7)), replace=False))
+    glob = sorted(set(range(nq)) - local)
+    for k in (1, 2, 3, 4):
+        got = hq.ClusterScheduler(gates, ctrls, diag, sorted(local), glob, k).ScheduleCluster()
+        best, cands = 0, []
+        for r in range(0, k + 1):
+            for cl in itertools.combinations(sorted(local), r):
+                n = len(_cluster_taken(gates, ctrls, diag, local, set(cl)))
+                if n > best:
+                    best, cands = n, [(r, cl)]
+                elif n == best and n:
+                    cands.append((r, cl))
+        if best:
+            rmin = min(r for r, _ in cands)
+            allowed = [_cluster_taken(gates, ctrls, diag, local, set(cl)) for r, cl in cands if r == rmin]
+            assert got in allowed, (seed, k, got, allowed)
+        else:
+            first = _cluster_taken(gates, ctrls, diag, local, local)
+            assert got == first[:1]
+
+
+def _swap_count(gates, ctrls, diag, local):
+    """Gates kept when `local` is the local set (swap_scheduler.h:73-88, in-order
+    rule): a gate touching a skipped gate's qubit is skipped; otherwise a
+    diagonal gate is taken and a non-diagonal one iff its targets are local."""
+    bad, n = set(), 0
+    for i, (t, c) in enumerate(zip(gates, ctrls)):
+        q = set(t) | set(c)
+        if not (q & bad) and (diag[i] or set(t) <= local):
+            n += 1
+        else:
+            bad |= q
+    return n
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_swap_scheduler_objective_brute_force(seed):
+    """With an unbounded split budget ScheduleSwap's backtracking is
+    exhaustive (swap_scheduler.cpp:90-172), so the local set it returns keeps
+    as many gates as the best set of at most num_locals qubits, found here by
+    enumerating every such set (single-qubit fusion off: every gate weighs 1)."""
+    import itertools
+    import hiqsimulator_amd as hq
+    rng = np.random.default_rng(100 + seed)
+    nq = 6
+    gates, ctrls, diag = _random_gate_list(rng, nq, 10)
+    for num_locals in (2, 3, 4):
+        got = hq.SwapScheduler(gates, ctrls, diag, 10 ** 6, num_locals, False).ScheduleSwap()
+        assert len(got) <= num_locals and len(set(got)) == len(got)
+        best = max(_swap_count(gates, ctrls, diag, set(l))
+                   for r in range(num_locals + 1) for l in itertools.combinations(range(nq), r))
+        assert _swap_count(gates, ctrls, diag, set(got)) == best, (seed, num_locals, got)
