@@ -203,6 +203,8 @@ struct Mfma16<double> {
      // real row r (0..15) of a block: amplitude offset in the block, imaginary part?
      static __device__ __forceinline__ int row_amp(int r) { return 2 * (r & 3) + (r >> 3); }
      static __device__ __forceinline__ bool row_im(int r) { return (r >> 2) & 1; }
+     // output row of accumulator element j on lanes 16q..16q+15
+     static __device__ __forceinline__ int out_row(int q, int j) { return q + 4 * j; }
 };
 template <>
 struct Mfma16<float> {
@@ -213,6 +215,7 @@ struct Mfma16<float> {
      }
      static __device__ __forceinline__ int row_amp(int r) { return r >> 1; }
      static __device__ __forceinline__ bool row_im(int r) { return r & 1; }
+     static __device__ __forceinline__ int out_row(int q, int j) { return 4 * q + j; }
 };
 
 
@@ -309,11 +312,111 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
      }
 }
 
+// The same pass for 4 <= K <= 5 with three real D x D products instead of
+// the 2D x 2D real form (Gauss's three-multiplication complex product):
+//   P1 = Re M . Re x,  P2 = Im M . Im x,  P3 = (Re M + Im M) . (Re x + Im x),
+//   out.re = P1 - P2,  out.im = P3 - P1 - P2,
+// 3 D^2 / 64 instructions per 16 groups instead of 4 D^2 / 64.  Here the
+// real rows of a 16-row block are the amplitudes 16 * rb + r in natural
+// order; instruction s takes input amplitude 4 * s + q from lane (q, c), so
+// one LDS load of a complex amplitude feeds three instructions, and lane
+// (q, c)'s accumulator element j is output amplitude 16 * rb + out_row(q, j)
+// of group c.  (For K = 3, D = 8 rows would fill half of each 16-row
+// instruction, so those ops keep the 2D x 2D form.)  The rounding error of
+// out.im grows from ~eps |M| |x| to ~2 eps (|Re M| + |Im M|)(|Re x| + |Im x|).
+template <int K, typename R, int NWV, int NCB, int N = kTileBits>
+__device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restrict__ mat,
+                                               const uint8_t (&pos)[5], const int w, const uint32_t cb0)
+{
+     static_assert(K >= 4 && K <= 5, "three-product MFMA pass needs 4 <= K <= 5");
+     using V = VT<R>;
+     using M = Mfma16<R>;
+     using acc_t = typename M::acc_t;
+     constexpr int D = 1 << K;
+     constexpr int RB = D / 16;  // 16-row blocks of each product
+     constexpr int T = D / 4;    // instructions per product and input amplitudes per lane
+     static_assert(NCB % NWV == 0, "column blocks must divide over the waves");
+     constexpr int CB = NCB / NWV;
+     constexpr bool kHoldA = RB == 1;  // K = 4: 3 T values per lane
+     const int lane = threadIdx.x & 63;
+     const int q = lane >> 4, c = lane & 15;
+     // A fragments of row block rb: lane (q, c) holds Re, Im and Re + Im of
+     // M[16 rb + c][4 t + q].
+     auto load_a = [&](int rb, R (&a)[3 * T]) {
+          const int o = rb * 16 + c;
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+               const V m = mat[o * D + 4 * t + q];
+               a[3 * t] = m.x;
+               a[3 * t + 1] = m.y;
+               a[3 * t + 2] = m.x + m.y;
+          }
+     };
+     R ahold[3 * T];
+     if constexpr (kHoldA) load_a(0, ahold);
+     uint32_t in_off[T], out_off[RB][4];
+#pragma unroll
+     for (int t = 0; t < T; ++t) in_off[t] = amp_offset<K>(4 * t + q, pos);
+#pragma unroll
+     for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out_off[rb][j] = amp_offset<K>(rb * 16 + M::out_row(q, j), pos);
+     constexpr int HB = CB < 2 ? CB : 2;
+#pragma unroll
+     for (int i0 = 0; i0 < CB; i0 += HB) {
+          uint32_t base[HB];
+          V x[HB][T];
+#pragma unroll
+          for (int i = 0; i < HB; ++i) {
+               uint32_t b = (cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16 + c;
+#pragma unroll
+               for (int l = 0; l < K; ++l) {
+                    const uint32_t lo = b & ((1u << pos[l]) - 1u);
+                    b = ((b ^ lo) << 1) | lo;
+               }
+               base[i] = b;
+#pragma unroll
+               for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
+          }
+#pragma unroll
+          for (int i = 0; i < HB; ++i) {
+               auto block = [&](const R (&a)[3 * T], int rb) {
+                    acc_t p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+#pragma unroll
+                    for (int t = 0; t < T; ++t) {
+                         p1 = M::step(a[3 * t], x[i][t].x, p1);
+                         p2 = M::step(a[3 * t + 1], x[i][t].y, p2);
+                         p3 = M::step(a[3 * t + 2], x[i][t].x + x[i][t].y, p3);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                         V y;
+                         y.x = p1[j] - p2[j];
+                         y.y = p3[j] - p1[j] - p2[j];
+                         psi[base[i] + out_off[rb][j]] = y;
+                    }
+               };
+#pragma unroll
+               for (int rb = 0; rb < RB; ++rb) {
+                    if constexpr (kHoldA) {
+                         block(ahold, rb);
+                    }
+                    else {
+                         R a[3 * T];
+                         load_a(rb, a);
+                         block(a, rb);
+                    }
+               }
+          }
+     }
+}
+
 template <int K, typename R, int NT, int N = kTileBits>
 __device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
 {
-     mfma_blocks<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos,
-                                                       __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), 0);
+     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+     if constexpr (K >= 4) mfma_blocks_3m<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos, w, 0);
+     else mfma_blocks<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos, w, 0);
 }
 
 // |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
