@@ -235,7 +235,7 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
         states[prec] = [b.get_state(i) for i in sample_idx]
         out[prec] = {"circuits_per_s": round(count / (ms * 1e-3), 1), "batch_ms": round(ms, 3),
                      "effective_gbs": round(st["amp_passes"] * 2 * ab / (ms * 1e-3) / 1e9, 1),
-                     "fused_passes": st["passes"], "wg_circuits": st["wg_circuits"],
+                     "hbm_passes": st["passes"], "wg_circuits": st["wg_circuits"],
                      "lds_circuits": st["lds_circuits"], "layer_launches": st["layer_launches"],
                      "arena_gib": round(st["arena_bytes"] / 2 ** 30, 2), "build_s": round(build_s, 2),
                      "plan": "autotuned table" if tune else "default (k=4, wg_max_qubits=16)",
