@@ -406,6 +406,12 @@ void Engine::apply_gate(Mat m, std::vector<int64_t> ids, const std::vector<int64
 void Engine::run()
 {
      DeviceScope ds(device_);
+     // A pending fusion wider than a pass can apply is rejected before its
+     // 2^n x 2^n matrix is built (the reference builds it first and fails
+     // with the same message; at 15 qubits that is 16 GiB of host work).
+     if (fusion_.num_qubits() > 5)
+          throw std::runtime_error("Run(): cannot apply " + std::to_string(fusion_.num_qubits())
+                                   + " qubits gate");
      Mat m;
      std::vector<int64_t> ids, ctrls;
      bool diag = false;

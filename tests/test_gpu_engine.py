@@ -144,6 +144,21 @@ def test_expectation_and_operator_vs_oracle(engine_mod, oracle_mod):
     assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
 
 
+def test_run_rejects_wide_pending_fusion(engine_mod):
+    """Gates left pending across 13 qubits: run() raises the reference's
+    "cannot apply" error before building the 2^13 x 2^13 fused matrix
+    (which would take 1 GiB and seconds of host work)."""
+    import time
+    sim = make_engine(1, 13, 5)
+    sim.allocate_qureg(list(range(13)), 0)
+    for q in range(13):
+        sim.apply_controlled_gate(H, [q], [])
+    t0 = time.perf_counter()
+    with pytest.raises(RuntimeError, match="cannot apply 13 qubits gate"):
+        sim.run()
+    assert time.perf_counter() - t0 < 2.0
+
+
 def test_permutations_keep_results(engine_mod, oracle_mod):
     """Random qubit permutations between circuit halves: state (read in the
     permuted order) and probabilities agree with the oracle, which also

