@@ -241,7 +241,9 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
                      "plan": "autotuned table" if tune else "default (k=4, wg_max_qubits=16)",
                      "wg_max_qubits": wg_max, "tile_min_qubits": tile_min,
                      "prog_min_qubits": prog_min, "tile_program_passes": st["tile_passes"],
-                     "tile_program_ops": st["tile_ops"]}
+                     "tile_program_ops": st["tile_ops"],
+                     # algorithmic flops of the tile ops run on the matrix cores
+                     "mfma_tflops": round(st["tile_mfma_flops"] / (ms * 1e-3) / 1e12, 2)}
         log_fn("batch %s: %s" % (prec, out[prec]))
         del b
     fids = [fidelity(a, b) for a, b in zip(states["c128"], states["c64"])]

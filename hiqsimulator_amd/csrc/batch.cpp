@@ -331,6 +331,7 @@ Batch::HostPlan Batch::plan_host(int workgroups)
                     tmat_off[c].push_back(mat_entries);
                     mat_entries += op.m.size();
                     stats_.kmax = std::max(stats_.kmax, op.k);
+                    if (op.k >= 3) stats_.tile_mfma_flops += uint64_t(8) << (op.k + circ_[c].n);
                }
                stats_.amp_passes += uint64_t(1) << circ_[c].n;
                stats_.tile_ops += t.ops.size();

@@ -227,6 +227,7 @@ struct BatchStats {
      uint64_t padded_launches = 0;  // layered launches whose 2-D grid has padding blocks
      uint64_t tile_passes = 0;      // HBM passes run as tile programs
      uint64_t tile_ops = 0;         // ops inside them
+     uint64_t tile_mfma_flops = 0;  // 8 * 2^k * 2^n summed over their ops of k >= 3
 };
 
 class Batch {

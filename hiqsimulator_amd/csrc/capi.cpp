@@ -718,12 +718,12 @@ int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len)
           need(b, "batch");
           if (len) need(out, "out");
           const hq::BatchStats& s = b->b.stats();
-          const uint64_t v[13] = {s.circuits,       s.passes,      s.wg_circuits,
+          const uint64_t v[14] = {s.circuits,       s.passes,      s.wg_circuits,
                                   s.lds_circuits,   s.layer_launches, s.arena_bytes,
                                   s.matrix_bytes,   s.amp_passes,  static_cast<uint64_t>(s.wg_grid),
                                   static_cast<uint64_t>(s.kmax), s.padded_launches, s.tile_passes,
-                                  s.tile_ops};
-          for (size_t i = 0; i < len && i < 13; ++i) out[i] = v[i];
+                                  s.tile_ops,       s.tile_mfma_flops};
+          for (size_t i = 0; i < len && i < 14; ++i) out[i] = v[i];
      });
 }
 

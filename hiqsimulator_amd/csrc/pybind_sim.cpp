@@ -652,15 +652,15 @@ public:
      }
      py::dict stats() const
      {
-          uint64_t v[13] = {0};
-          ok(hiqsim_batch_stats(h_, v, 13));
-          static const char* names[13] = {"circuits",      "passes",         "wg_circuits",
+          uint64_t v[14] = {0};
+          ok(hiqsim_batch_stats(h_, v, 14));
+          static const char* names[14] = {"circuits",      "passes",         "wg_circuits",
                                           "lds_circuits",  "layer_launches", "arena_bytes",
                                           "matrix_bytes",  "amp_passes",     "wg_grid",
                                           "kmax",          "padded_launches", "tile_passes",
-                                          "tile_ops"};
+                                          "tile_ops",      "tile_mfma_flops"};
           py::dict d;
-          for (int i = 0; i < 13; ++i) d[names[i]] = v[i];
+          for (int i = 0; i < 14; ++i) d[names[i]] = v[i];
           return d;
      }
 

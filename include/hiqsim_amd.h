@@ -287,11 +287,12 @@ int hiqsim_batch_num_circuits(hiqsim_batch_t* b, int64_t* n);
 int hiqsim_batch_num_qubits(hiqsim_batch_t* b, int64_t index, int* n_qubits);
 /* Final state of circuit `index`: 2^n interleaved complex128 values. */
 int hiqsim_batch_get_state(hiqsim_batch_t* b, int64_t index, double* out);
-/* out[0..12] = circuits, HBM passes, workgroup-owned circuits, LDS-resident
+/* out[0..13] = circuits, HBM passes, workgroup-owned circuits, LDS-resident
  * circuits, layered launches, arena bytes, matrix bytes, amplitude-passes
  * (sum over HBM passes of 2^n), persistent workgroups, widest fusion width,
  * layered launches whose 2-D grid carries padding blocks, tile-program
- * passes, ops inside them. */
+ * passes, ops inside them, algorithmic flops of those ops of 3+ qubits
+ * (8 * 2^k * 2^n each: the matrix-core ops of the default op engine). */
 int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len);
 
 /* ---- request loop (continuous batching) ---------------------------------

@@ -47,11 +47,20 @@ def test_prepare_tile_programs(prog_min):
     st = b.stats()
     progs = [(n, g) for n, g in circuits if n > 11 and n >= prog_min]
     assert (st["tile_passes"] > 0) == bool(progs)
-    expect = 0
+    expect = flops = 0
     for n, g in progs:
-        po, _, _ = hq._cppsim_amd.plan_tile_passes([x[1] for x in g], [x[2] for x in g], n, 4)
+        po, oo, _ = hq._cppsim_amd.plan_tile_passes([x[1] for x in g], [x[2] for x in g], n, 4)
         expect += max(po) + 1 if po else 0
+        # each op acts on the union of its gates' qubits (controls folded in)
+        ops = {}
+        for i, key in enumerate(zip(po, oo)):
+            ops.setdefault(key, set()).update(list(g[i][1]) + list(g[i][2]))
+        flops += sum(8 * 2 ** len(q) * 2 ** n for q in ops.values() if len(q) >= 3)
     assert st["tile_passes"] == expect
+    assert st["tile_ops"] == sum(
+        len(set(zip(*hq._cppsim_amd.plan_tile_passes([x[1] for x in g], [x[2] for x in g], n, 4)[:2])))
+        for n, g in progs)
+    assert st["tile_mfma_flops"] == flops
 
 
 def test_prepare_large_launches():
