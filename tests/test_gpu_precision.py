@@ -1,4 +1,4 @@
-"""Precision (config[2]): QFT-like circuit on a random normalised state, in
+"""Precision of the 32-qubit QFT workload: QFT-like circuit on a random normalised state, in
 complex128 and complex64, against FP64 references.
 
 * 26 qubits: the final state against numpy's FFT (the QFT-like circuit is
