@@ -315,7 +315,7 @@ def main():
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-rates", default="20,60")
+    ap.add_argument("--serve-rates", default="20,60,120")
     ap.add_argument("--block-swizzle", type=int, default=None,
                     help="low block-index bits reversed in the streaming passes (library default 8)")
     ap.add_argument("--block-swizzle-wide", type=int, default=None,
