@@ -103,6 +103,31 @@ def test_tiled_after_permutation_and_pending_gates(engine_mod, oracle_mod):
     assert np.abs(got - ref[src]).max() <= 1e-12
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_tiled_round_trip_30q(engine_mod, precision):
+    """Size-independent property at the benchmark size: the 30-qubit, 40-layer
+    benchmark circuit with tile passes, then its inverse, returns a random
+    state to itself (compared on the device in FP64)."""
+    n = 30
+    gates = random_layers_circuit(n, 40, 1234)
+    inv = [(m.conj().T, t, c) for (m, t, c) in reversed(gates)]
+    print("tiled round trip 30q %s" % precision, flush=True)
+    g = engine_mod.SimulatorMPI(1, n, 4, precision)
+    g.allocate_qureg(list(range(n)), 0)
+    g.init_random_state(30)
+    g.snapshot()
+    passes = g.apply_tiled(gates, 4) + g.apply_tiled(inv, 4)
+    r = g.compare_snapshot()
+    print("tiled round trip 30q %s: %d passes, max-abs %.3e, 1 - fidelity %.3e"
+          % (precision, passes, r["max_abs_diff"], 1 - r["fidelity"]), flush=True)
+    assert passes < 2 * 196  # fewer HBM passes than the fused form
+    if precision == "c128":
+        assert r["max_abs_diff"] <= 1e-12
+        assert 1 - r["fidelity"] <= 1e-12
+    else:
+        assert r["fidelity"] >= 1 - 1e-6
+
+
 def test_tiled_c64_fidelity(engine_mod, oracle_mod):
     n = 18
     g = circuit(n, 12, 9)
