@@ -27,6 +27,7 @@ def mixed_circuits(seed):
             g.append((haar_unitary(4, rng), [n - 1, 0], [1]))              # controlled 2-qubit
         out.append((n, g))
     out.append((4, []))  # no gates: stays |0000>
+    out.append((12, []))  # no gates on the layered and tile-program paths either
     return out
 
 

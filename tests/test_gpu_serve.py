@@ -69,6 +69,30 @@ def test_same_alone_and_in_mix(engine_mod, oracle_mod, precision, prog_min):
             assert np.abs(mix[i] - ref).max() <= tol, (i, np.abs(mix[i] - ref).max())
 
 
+@pytest.mark.parametrize("prog_min", [31, 12])
+def test_edge_requests(engine_mod, prog_min):
+    """Requests with no gates, 1-qubit requests and a 12-qubit one with a
+    single gate, on the fused-pass and tile-program classes: basis states
+    give one outcome per shot, and the expectations are exact."""
+    from hiqsimulator_amd.circuits import H, X
+    reqs = [
+        dict(arrival=0.0, n=1, gates=[], kind="samples", shots=64, seed=5, terms=[]),
+        dict(arrival=0.0, n=13, gates=[], kind="samples", shots=64, seed=6, terms=[]),
+        dict(arrival=0.0, n=12, gates=[(X, [11], [])], kind="samples", shots=64, seed=7, terms=[]),
+        dict(arrival=0.0, n=1, gates=[(H, [0], [])], kind="expectation", shots=0, seed=0,
+             terms=[(0, 1), (1, 0), (1, 1)]),  # Z, X, Y on |+>
+        dict(arrival=0.0, n=13, gates=[], kind="expectation", shots=0, seed=0,
+             terms=[(0, 1 << 12), (1 << 12, 0)]),  # Z, X on |0>
+    ]
+    print("serve edge requests prog_min", prog_min, flush=True)
+    out = serve_all(reqs, "c128", passes_per_step=4, prog_min_qubits=prog_min)
+    assert np.array_equal(out[0], np.zeros(64, dtype=np.uint64))
+    assert np.array_equal(out[1], np.zeros(64, dtype=np.uint64))
+    assert np.array_equal(out[2], np.full(64, 1 << 11, dtype=np.uint64))
+    assert np.abs(np.asarray(out[3]) - np.array([0.0, 1.0, 0.0])).max() <= 1e-12
+    assert np.abs(np.asarray(out[4]) - np.array([1.0, 0.0])).max() <= 1e-12
+
+
 def test_samples_match_engine_sampler(engine_mod):
     reqs = [r for r in small_requests(4) if r["kind"] == "samples"]
     got = serve_all(reqs, "c128", prog_min_qubits=31)
