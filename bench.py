@@ -119,6 +119,24 @@ def run_step(sim, step, op_width=4):
     return passes
 
 
+def inverse_step(step, start_order):
+    """The exact inverse of a step that starts with the qubits in
+    start_order: passes inverted (gates reversed and conjugate-transposed) in
+    reverse order, and each permutation undone by permuting back to the order
+    before it."""
+    orders = [list(start_order)] + [list(p) for kind, p in step if kind == "perm"]
+    out, j = [], len(orders) - 1
+    for kind, payload in reversed(step):
+        if kind == "pass":
+            out.append(("pass", [(m.conj().T, t, c) for (m, t, c) in reversed(payload)]))
+        elif kind == "perm":
+            j -= 1
+            out.append(("perm", orders[j]))
+        else:
+            raise ValueError("inverse_step: unexpected entry %r" % kind)
+    return out
+
+
 def kernel_sweep(hq, n, reps=6):
     """Per-pass GB/s of the dense kernel for k = 1..5 at random positions."""
     from hiqsimulator_amd.circuits import haar_unitary
@@ -305,6 +323,8 @@ def main():
     ap.add_argument("--planner", choices=["greedy", "inorder"], default="greedy",
                     help="greedy: the GreedyScheduler/ClusterScheduler loop; inorder: program order")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the step-and-inverse error check of the headline circuit")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-tile", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
@@ -376,6 +396,19 @@ def main():
     times = {name: sim.kernel_time(i) for i, name in enumerate(hq.KERNEL_CLASSES)}
     sim.set_timing(False)
     norm = sim.get_probability([], [])
+    # Error of the timed step: one more step and its exact inverse from the
+    # current state, compared on the device with a snapshot in FP64.
+    parity = None
+    if not args.no_parity:
+        start = list(sim.get_qubits_ids())
+        sim.snapshot()
+        run_step(sim, step)
+        run_step(sim, inverse_step(step, start))
+        r = sim.compare_snapshot()
+        parity = {"workload": "one step and its inverse from the state after the timed steps",
+                  "max_abs": float(r["max_abs_diff"]), "one_minus_fidelity": float(1.0 - r["fidelity"]),
+                  "tolerance": 1e-12}
+        log("parity: %s" % parity)
     bytes_step = 32.0 * 2 ** n * (n_pass + n_perm)
     ms_step = el / args.steps * 1e3
     value = bytes_step * args.steps / el / 1e9
@@ -480,6 +513,7 @@ def main():
         "k_sweep_gbs": sweep,
         "tile_planner": tile,
         "cpu_baseline": cpu,
+        "parity": parity,
         "precision": precision,
         "batch": batch,
         "serving": serving,
