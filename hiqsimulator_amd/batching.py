@@ -206,7 +206,9 @@ def workload_bytes(circuits, precision):
 def main(argv=None):
     """``python -m hiqsimulator_amd.batching``: builds the batch workload with
     the stored tuning table and replays it; prints the plan and the replay
-    times (a short driver for profiling the batch kernels on their own)."""
+    times (a short driver for profiling the batch kernels on their own).
+    With ``--autotune`` it first re-measures the table for the precision
+    (qubit counts 10..22) and stores it."""
     import argparse
     import json
     ap = argparse.ArgumentParser(description=main.__doc__)
@@ -217,10 +219,18 @@ def main(argv=None):
     ap.add_argument("--tile-mfma", type=int, default=None,
                     help="tile-program op engine: 1 ops of 3+ qubits on the matrix cores, 0 all on "
                          "the vector units")
+    ap.add_argument("--autotune", action="store_true",
+                    help="re-measure and store the tuning table before replaying")
     a = ap.parse_args(argv)
     from . import _cppsim_amd
     if a.tile_mfma is not None:
         _cppsim_amd.set_tuning("tile_mfma", a.tile_mfma)
+    if a.autotune:
+        res = autotune(range(10, 23), a.precision, amp_budget=1 << 23, reps=2,
+                       log=lambda m: print(m, flush=True))
+        save_tuning(res, a.precision)
+        print("autotune %s: %s -> %s" % (a.precision, json.dumps(
+            {k: v for k, v in res.items() if k != "probe_ms"}), tuning_path(a.precision)), flush=True)
     circ = batch_workload(a.circuits, a.seed)
     t = load_tuning(a.precision)
     if t is None:
