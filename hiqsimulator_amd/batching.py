@@ -197,12 +197,6 @@ def fidelity(a, b):
     return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
 
 
-def workload_bytes(circuits, precision):
-    """State bytes of a list of circuits."""
-    ab = 16 if precision == "c128" else 8
-    return sum(ab << n for n, _ in circuits)
-
-
 def main(argv=None):
     """``python -m hiqsimulator_amd.batching``: builds the batch workload with
     the stored tuning table and replays it; prints the plan and the replay
