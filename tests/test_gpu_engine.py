@@ -261,6 +261,34 @@ def test_benchmark_circuit_round_trip_30q(engine_mod):
     assert 1 - r["fidelity"] <= 1e-12
 
 
+def test_measurement_and_sampling_30q(engine_mod):
+    """Size-independent checks at 30 qubits (16 GiB complex128), with the
+    top bit involved: H on qubits 0, 15 and 29 of |0..0> gives 8 equally
+    likely outcomes.  Probabilities are 1/8 each; 8192 samples fall only on
+    those outcomes with frequencies that pass a chi-square test; measuring
+    qubit 29 collapses it (probability 1 afterwards) and keeps the norm."""
+    n = 30
+    print("measurement and sampling 30q", flush=True)
+    g = make_engine(77, n, 4)
+    g.allocate_qureg(list(range(n)), 0)
+    for q in (0, 15, 29):
+        g.apply_controlled_gate(H, [q], [])
+        g.run()
+    for bits in range(8):
+        vals = [(bits >> i) & 1 for i in range(3)]
+        assert abs(g.get_probability(vals, [0, 15, 29]) - 0.125) < 1e-12
+    s = np.asarray(g.sample(8192, 2024), dtype=np.uint64)
+    allowed = {(b & 1) | ((b >> 1 & 1) << 15) | ((b >> 2 & 1) << 29) for b in range(8)}
+    assert set(int(x) for x in np.unique(s)) <= allowed
+    counts = np.array([np.count_nonzero(s == a) for a in sorted(allowed)])
+    chi2 = float(((counts - 1024.0) ** 2 / 1024.0).sum())
+    assert chi2 < 24.3  # 7 degrees of freedom, p = 0.001
+    (bit,) = g.measure_qubits([29])
+    assert abs(g.get_probability([bit], [29]) - 1.0) < 1e-12
+    assert abs(g.get_probability([], []) - 1.0) < 1e-12
+    assert abs(g.get_probability([0, 0], [0, 15]) - 0.25) < 1e-12
+
+
 def test_snapshot_restore(engine_mod):
     n = 18
     g = make_engine(1, n, 4)
