@@ -289,6 +289,32 @@ def test_measurement_and_sampling_30q(engine_mod):
     assert abs(g.get_probability([0, 0], [0, 15]) - 0.25) < 1e-12
 
 
+def test_allocation_and_deallocation_30q(engine_mod):
+    """Allocation and deallocation at 30 qubits: a 29-qubit state grows by
+    one qubit (16 GiB), the new qubit is entangled with qubit 28 and
+    measured, and deallocating it shrinks the state back with its norm and
+    the other qubit's collapsed value kept."""
+    print("allocation and deallocation 30q", flush=True)
+    g = make_engine(5, 30, 4)
+    g.allocate_qureg(list(range(29)), 0)
+    g.apply_controlled_gate(H, [28], [])
+    g.run()
+    g.allocate_qubit(29)
+    g.apply_controlled_gate(X, [29], [28])  # (|00> + |11>) / sqrt(2) on 28, 29
+    g.run()
+    assert abs(g.get_probability([1, 1], [28, 29]) - 0.5) < 1e-12
+    assert abs(g.get_probability([0, 1], [28, 29])) < 1e-12
+    with pytest.raises(RuntimeError):
+        g.deallocate_qubit(29)  # entangled
+    (bit,) = g.measure_qubits([28])
+    assert abs(g.get_probability([bit], [29]) - 1.0) < 1e-12
+    g.deallocate_qubit(29)
+    assert len(g.get_qubits_ids()) == 29
+    assert abs(g.get_probability([], []) - 1.0) < 1e-12
+    assert abs(g.get_probability([bit], [28]) - 1.0) < 1e-12
+    assert abs(g.get_amplitude([0] * 28 + [bit], list(range(29))) - 1.0) < 1e-12
+
+
 def test_snapshot_restore(engine_mod):
     n = 18
     g = make_engine(1, n, 4)
