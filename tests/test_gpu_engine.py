@@ -315,6 +315,34 @@ def test_allocation_and_deallocation_30q(engine_mod):
     assert abs(g.get_amplitude([0] * 28 + [bit], list(range(29))) - 1.0) < 1e-12
 
 
+def test_ghz_expectations_30q(engine_mod):
+    """Pauli expectations and a Pauli operator at 30 qubits against the
+    closed forms of the GHZ state (|0..0> + |1..1>) / sqrt(2):
+    <Z0 Z29> = 1, <X..X> = 1, <Y0 Y1 X..X> = -1, <Z0> = 0, and X on every
+    qubit maps the state to itself."""
+    n = 30
+    print("ghz expectations 30q", flush=True)
+    g = make_engine(3, n, 4)
+    g.allocate_qureg(list(range(n)), 0)
+    g.apply_controlled_gate(H, [0], [])
+    g.run()
+    for q in range(1, n):
+        g.apply_controlled_gate(X, [q], [q - 1])
+        g.run()
+    ids = list(range(n))
+
+    def ev(term):
+        return g.get_expectation_value([(term, 1.0)], ids)
+
+    assert abs(ev([(0, "Z"), (29, "Z")]) - 1.0) < 1e-12
+    assert abs(ev([(q, "X") for q in range(n)]) - 1.0) < 1e-12
+    assert abs(ev([(0, "Y"), (1, "Y")] + [(q, "X") for q in range(2, n)]) + 1.0) < 1e-12
+    assert abs(ev([(0, "Z")])) < 1e-12
+    g.apply_qubit_operator([([(q, "X") for q in range(n)], 1.0)], ids)
+    assert abs(g.get_probability([1] * n, ids) - 0.5) < 1e-12
+    assert abs(g.get_amplitude([0] * n, ids) - 1.0 / math.sqrt(2.0)) < 1e-12
+
+
 def test_snapshot_restore(engine_mod):
     n = 18
     g = make_engine(1, n, 4)
