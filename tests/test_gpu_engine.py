@@ -343,6 +343,28 @@ def test_ghz_expectations_30q(engine_mod):
     assert abs(g.get_amplitude([0] * n, ids) - 1.0 / math.sqrt(2.0)) < 1e-12
 
 
+def test_time_evolution_30q(engine_mod):
+    """exp(-i t H) at 30 qubits against a closed form: H = c1 Z0 + c2 X0 on
+    |0>, controlled by qubit 29.  With qubit 29 set, P(qubit 0 = 1) =
+    c2^2 / (c1^2 + c2^2) * sin^2(t * sqrt(c1^2 + c2^2)); with it clear the
+    state is unchanged."""
+    n = 30
+    c1, c2, t = 0.3, 0.7, 1.9
+    w = math.sqrt(c1 * c1 + c2 * c2)
+    expect = (c2 / w) ** 2 * math.sin(t * w) ** 2
+    print("time evolution 30q", flush=True)
+    for ctrl in (1, 0):
+        g = make_engine(4, n, 4)
+        g.allocate_qureg(list(range(n)), 0)
+        if ctrl:
+            g.apply_controlled_gate(X, [29], [])
+            g.run()
+        g.emulate_time_evolution([([(0, "Z")], c1), ([(0, "X")], c2)], t, [0], [29])
+        p1 = g.get_probability([1], [0])
+        assert abs(p1 - (expect if ctrl else 0.0)) < 1e-10, (ctrl, p1, expect)
+        assert abs(g.get_probability([], []) - 1.0) < 1e-10
+
+
 def test_snapshot_restore(engine_mod):
     n = 18
     g = make_engine(1, n, 4)
