@@ -594,6 +594,10 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                if (value < 0 || value > 1) throw std::runtime_error("tile_mfma must be 0 or 1");
                hq::set_tile_mfma(static_cast<int>(value));
           }
+          else if (n == "dense5_tile") {
+               if (value < 0 || value > 1) throw std::runtime_error("dense5_tile must be 0 or 1");
+               hq::set_dense5_tile(static_cast<int>(value));
+          }
           else
                throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
@@ -609,6 +613,7 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           else if (n == "block_swizzle") *value = hq::get_block_swizzle();
           else if (n == "block_swizzle_wide") *value = hq::get_block_swizzle_wide();
           else if (n == "tile_mfma") *value = hq::get_tile_mfma();
+          else if (n == "dense5_tile") *value = hq::get_dense5_tile();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

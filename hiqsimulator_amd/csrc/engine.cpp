@@ -151,8 +151,9 @@ void Engine::swap_tmp()
      std::swap(cap_state_, cap_tmp_);
 }
 
-const void* Engine::upload_matrix(const Mat& m)
+const void* Engine::upload_matrix(const Mat& m, const void* desc, size_t desc_bytes)
 {
+     if (desc_bytes > kSlotBytes - kMatBytes) throw std::runtime_error("upload_matrix(): descriptor too large");
      const int slot = ring_next_;
      ring_next_ = (ring_next_ + 1) % kRing;
      if (ring_used_[slot]) check(hipEventSynchronize(ring_ev_[slot]), "hipEventSynchronize");
@@ -174,6 +175,10 @@ const void* Engine::upload_matrix(const Mat& m)
                o[2 * i + 1] = static_cast<float>(m.a[i].imag());
           }
           bytes = n * 8;
+     }
+     if (desc_bytes) {
+          std::memcpy(h + kMatBytes, desc, desc_bytes);
+          bytes = kMatBytes + desc_bytes;
      }
      char* d = d_ring_ + slot * kSlotBytes;
      check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(matrix)");
@@ -468,12 +473,67 @@ void Engine::dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint6
           check(launch_diag(d_state_, prec_, n_qubits(), dv, d, stream_), "diag pass");
           end_timed();
      }
-     else {
+     else if (!dense5_as_tile(m, pos, ctrlmask)) {
           const void* dm = upload_matrix(m);
           begin_timed(kDense1 + d.k - 1);
           check(launch_dense(d_state_, prec_, n_qubits(), dm, d, stream_), "dense pass");
           end_timed();
      }
+}
+
+// A complex128 5-qubit pass without controls, on 12 or more qubits, runs as
+// a tile pass with one op: the tile holds bits 0..5 plus the targets at or
+// above bit 6 (filled up with the lowest other bits), and the op runs on the
+// matrix cores as three real products.  The dense forms keep the 32-amplitude
+// group in registers on the vector units; on a 30-qubit state the tile pass
+// takes 6.5 ms against 7.8 ms.  Timed as a 5-qubit dense pass.
+bool Engine::dense5_as_tile(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask)
+{
+     const int n = n_qubits();
+     if (prec_ != Prec::C128 || pos.size() != 5 || ctrlmask != 0 || n < kTileBits || !get_dense5_tile())
+          return false;
+     // matrix bit l' <-> the l'-th lowest position (the tile op wants ascending ranks)
+     int order[5] = {0, 1, 2, 3, 4};
+     std::sort(order, order + 5, [&](int a, int b) { return pos[a] < pos[b]; });
+     Mat ms(32);
+     auto old_index = [&](size_t i) {
+          size_t o = 0;
+          for (int l = 0; l < 5; ++l)
+               if ((i >> l) & 1) o |= size_t(1) << order[l];
+          return o;
+     };
+     for (size_t i = 0; i < 32; ++i)
+          for (size_t j = 0; j < 32; ++j) ms(i, j) = m(old_index(i), old_index(j));
+     uint64_t high = 0;
+     for (auto p: pos)
+          if (p >= 6) high |= uint64_t(1) << p;
+     for (int p = 6; p < n && __builtin_popcountll(high) < kTileHigh; ++p) high |= uint64_t(1) << p;
+     TileHigh th{};
+     int rank[64];
+     for (int p = 0; p < 64; ++p) rank[p] = p < 6 ? p : -1;
+     int jh = 0;
+     for (int p = 6; p < n; ++p)
+          if ((high >> p) & 1) {
+               th.hpos[jh] = static_cast<uint8_t>(p);
+               rank[p] = 6 + jh;
+               ++jh;
+          }
+     if (jh != kTileHigh) throw std::runtime_error("Run(): internal tile layout error");
+     TileOp op{};
+     op.mat_off = 0;
+     op.k = 5;
+     for (int l = 0; l < 5; ++l) {
+          const int r = rank[pos[order[l]]];
+          if (r < 0) throw std::runtime_error("Run(): internal tile layout error");
+          op.rpos[l] = static_cast<uint8_t>(r);
+     }
+     const char* dm = static_cast<const char*>(upload_matrix(ms, &op, sizeof(TileOp)));
+     begin_timed(kDense5);
+     check(launch_tile_pass(d_state_, prec_, n, th, reinterpret_cast<const TileOp*>(dm + kMatBytes), 1, 5, dm,
+                            stream_),
+           "dense pass (tile)");
+     end_timed();
+     return true;
 }
 
 void Engine::apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,

@@ -160,9 +160,10 @@ private:
      void ensure_state(uint64_t n_amps);
      void ensure_tmp(uint64_t n_amps);
      void swap_tmp();
-     const void* upload_matrix(const Mat& m);
+     const void* upload_matrix(const Mat& m, const void* desc = nullptr, size_t desc_bytes = 0);
      void dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
                         bool diag);
+     bool dense5_as_tile(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask);
      double masked_norm(uint64_t msk, uint64_t val);
      void mask_of(const std::vector<int64_t>& ids, const std::vector<uint8_t>& bits,
                   uint64_t& msk, uint64_t& val) const;
@@ -192,8 +193,11 @@ private:
      uint64_t n_amps_ = 1;
 
      // Fused-matrix upload ring: pinned host slots copied into device slots.
+     // A slot holds a 32 x 32 complex128 matrix and, after it, the one-op
+     // tile descriptor of a 5-qubit pass run as a tile pass.
      static constexpr int kRing = 64;
-     static constexpr size_t kSlotBytes = 32 * 32 * 16;
+     static constexpr size_t kMatBytes = 32 * 32 * 16;
+     static constexpr size_t kSlotBytes = kMatBytes + 256;
      char* h_ring_ = nullptr;
      char* d_ring_ = nullptr;
      hipEvent_t ring_ev_[kRing] = {};

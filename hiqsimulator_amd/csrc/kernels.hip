@@ -598,6 +598,7 @@ int g_high_target = 64;
 int g_swizzle_bits = 8;
 int g_swizzle_bits_wide = 6;
 int g_tile_mfma = 1;
+int g_dense5_tile = 1;
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -764,6 +765,8 @@ int get_block_swizzle_wide() { return g_swizzle_bits_wide; }
 int get_block_swizzle_for(int k) { return k >= 4 ? g_swizzle_bits_wide : g_swizzle_bits; }
 void set_tile_mfma(int mode) { g_tile_mfma = mode ? 1 : 0; }
 int get_tile_mfma() { return g_tile_mfma; }
+void set_dense5_tile(int on) { g_dense5_tile = on ? 1 : 0; }
+int get_dense5_tile() { return g_dense5_tile; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -55,6 +55,10 @@ int get_block_swizzle_for(int k);
 // qubits on the matrix cores (default).
 void set_tile_mfma(int mode);
 int get_tile_mfma();
+// Complex128 5-qubit passes without controls on 12+ qubits: 1 (default) as a
+// one-op tile pass on the matrix cores; 0 the dense pass forms.
+void set_dense5_tile(int on);
+int get_dense5_tile();
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

@@ -238,7 +238,10 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *   "block_swizzle_wide": the same for dense passes of k >= 4 (default 6);
  *   "tile_mfma":          op engine of tile programs: 1 (default) ops of 3+
  *                         qubits on the matrix cores; 0 every op on the
- *                         vector units.
+ *                         vector units;
+ *   "dense5_tile":        complex128 5-qubit passes without controls on 12+
+ *                         qubits: 1 (default) as a one-op tile pass on the
+ *                         matrix cores; 0 the dense pass forms.
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);
 int hiqsim_get_tuning(const char* name, int64_t* value);
