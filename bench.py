@@ -137,8 +137,9 @@ def inverse_step(step, start_order):
     return out
 
 
-def kernel_sweep(hq, n, reps=6):
-    """Per-pass GB/s of the dense kernel for k = 1..5 at random positions."""
+def kernel_sweep(hq, n, reps=24):
+    """Per-pass GB/s of the dense kernel for k = 1..5, averaged over `reps`
+    random placements (a pass's time depends on where its targets lie)."""
     from hiqsimulator_amd.circuits import haar_unitary
     rng = np.random.default_rng(5)
     sim = hq.SimulatorMPI(1, n, 5)
