@@ -343,6 +343,8 @@ def main():
                     help="the same for dense passes of k >= 4 (library default 6)")
     ap.add_argument("--tile-high-target", type=int, default=None,
                     help="dense passes with every target at or above this bit use the direct form")
+    ap.add_argument("--direct-block", type=int, default=None,
+                    help="threads per block of the direct dense form of 4-qubit passes (256, 512, 1024)")
     ap.add_argument("--tile-mfma", type=int, default=None,
                     help="tile-program op engine: 1 ops of 3+ qubits on the matrix cores, 0 all on "
                          "the vector units")
@@ -361,10 +363,12 @@ def main():
         hq._cppsim_amd.set_tuning("block_swizzle_wide", args.block_swizzle_wide)
     if args.tile_high_target is not None:
         hq._cppsim_amd.set_tuning("tile_high_target", args.tile_high_target)
+    if args.direct_block is not None:
+        hq._cppsim_amd.set_tuning("direct_block", args.direct_block)
     if args.tile_mfma is not None:
         hq._cppsim_amd.set_tuning("tile_mfma", args.tile_mfma)
     tuning = {name: hq._cppsim_amd.get_tuning(name)
-              for name in ("block_swizzle", "block_swizzle_wide", "tile_high_target",
+              for name in ("block_swizzle", "block_swizzle_wide", "tile_high_target", "tile_high_target_wide", "direct_block",
                            "tile_mfma")}
     n, k = args.qubits, args.k
     t0 = time.perf_counter()

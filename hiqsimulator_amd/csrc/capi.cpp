@@ -598,6 +598,15 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                if (value < 0 || value > 1) throw std::runtime_error("dense5_tile must be 0 or 1");
                hq::set_dense5_tile(static_cast<int>(value));
           }
+          else if (n == "tile_high_target_wide") {
+               if (value < 0 || value > 64) throw std::runtime_error("tile_high_target_wide must be 0..64");
+               hq::set_tile_high_target_wide(static_cast<int>(value));
+          }
+          else if (n == "direct_block") {
+               if (value != 256 && value != 512 && value != 1024)
+                    throw std::runtime_error("direct_block must be 256, 512 or 1024");
+               hq::set_direct_block(static_cast<int>(value));
+          }
           else
                throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
@@ -614,6 +623,8 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           else if (n == "block_swizzle_wide") *value = hq::get_block_swizzle_wide();
           else if (n == "tile_mfma") *value = hq::get_tile_mfma();
           else if (n == "dense5_tile") *value = hq::get_dense5_tile();
+          else if (n == "direct_block") *value = hq::get_direct_block();
+          else if (n == "tile_high_target_wide") *value = hq::get_tile_high_target_wide();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

@@ -32,8 +32,8 @@ namespace {
 // psi[I + off_i] = sum_j m[i][j] v_j.  GPT groups per lane are loaded before
 // any is computed so that GPT * 2^K loads are in flight per lane.
 // ---------------------------------------------------------------------------
-template <int K, typename R, int GPT, bool NT>
-__global__ void __launch_bounds__(kBlock)
+template <int K, typename R, int GPT, bool NT, int BS = kBlock>
+__global__ void __launch_bounds__(BS)
     k_dense_direct(typename C2<R>::T* __restrict__ psi, const typename C2<R>::T* __restrict__ mat,
                    const DenseDesc d, const uint64_t ngroups, const int swz)
 {
@@ -48,12 +48,12 @@ __global__ void __launch_bounds__(kBlock)
                if ((j >> l) & 1) o += 1ull << d.pos[l];
           off[j] = o;
      }
-     const uint64_t t0 = swizzle_block(block_linear(), swz) * (kBlock * GPT) + threadIdx.x;
+     const uint64_t t0 = swizzle_block(block_linear(), swz) * (BS * GPT) + threadIdx.x;
      uint64_t I[GPT];
      V v[GPT][D];
 #pragma unroll
      for (int g = 0; g < GPT; ++g) {
-          const uint64_t t = t0 + static_cast<uint64_t>(g) * kBlock;
+          const uint64_t t = t0 + static_cast<uint64_t>(g) * BS;
           I[g] = insert_zeros(t < ngroups ? t : 0, d.ins) | d.ctrlmask;
           if (t < ngroups) {
 #pragma unroll
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kBlock)
      }
 #pragma unroll
      for (int g = 0; g < GPT; ++g) {
-          if (t0 + static_cast<uint64_t>(g) * kBlock >= ngroups) continue;
+          if (t0 + static_cast<uint64_t>(g) * BS >= ngroups) continue;
           V* const p = psi + I[g];
           group_matvec<K, V, R>(mat, v[g], [&](int i, V o) { st<NT>(p + off[i], o); });
      }
@@ -599,6 +599,11 @@ int g_swizzle_bits = 8;
 int g_swizzle_bits_wide = 6;
 int g_tile_mfma = 1;
 int g_dense5_tile = 1;
+// 4-qubit passes with every target at bit 13 or above take the direct form
+// in 1024-thread blocks (16 KiB contiguous per target row): 6.34 against
+// 6.63 ms in the tile form on a 30-qubit state, 32 such placements.
+int g_high_target_wide = 13;
+int g_direct_block = 1024;
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -608,6 +613,24 @@ int swizzle_bits_for(uint64_t blocks, int k)
      const int bits = get_block_swizzle_for(k);
      if (bits <= 0 || bits > 16) return 0;
      return (blocks % (1ull << bits)) == 0 ? bits : 0;
+}
+
+// Direct form in blocks of BS threads (each block covers BS consecutive
+// groups, so BS * sizeof(amplitude) contiguous bytes per target row).
+template <int K, typename R, int GPT, int BS>
+hipError_t dense_direct_launch(void* psi, const void* mat, const DenseDesc& d, uint64_t ngroups, bool nt,
+                               hipStream_t s)
+{
+     using V = typename C2<R>::T;
+     const uint64_t dblocks = (ngroups + BS * GPT - 1) / (BS * GPT);
+     const int swz = (ngroups % (BS * GPT)) == 0 ? swizzle_bits_for(dblocks, K) : 0;
+     if (nt)
+          hipLaunchKernelGGL((k_dense_direct<K, R, GPT, true, BS>), grid_2d(dblocks), dim3(BS), 0, s,
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups, swz);
+     else
+          hipLaunchKernelGGL((k_dense_direct<K, R, GPT, false, BS>), grid_2d(dblocks), dim3(BS), 0, s,
+                             static_cast<V*>(psi), static_cast<const V*>(mat), d, ngroups, swz);
+     return hipGetLastError();
 }
 
 template <int K, typename R>
@@ -633,7 +656,7 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      // occupancy; the direct form is faster there unless bit 0 (or, for
      // complex64, bit 0 or 1) is a target.
      const bool tile_pays = K < 5 || min_t <= (sizeof(V) == 16 ? 0 : 1);
-     if (tileable && tile_pays && min_t < g_high_target) {
+     if (tileable && tile_pays && min_t < (K == 4 ? g_high_target_wide : g_high_target)) {
           TileDesc L{};
           uint64_t lanes = 0;
           int nb = 0;
@@ -671,6 +694,10 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      constexpr int GPT = K >= 4 ? 1 : (K == 3 ? 4 : (K == 2 ? 8 : 16));
      const int line_bits = sizeof(V) == 16 ? 3 : 4;
      const bool nt = (ins_mask & ((1ull << line_bits) - 1)) == 0;
+     if constexpr (K == 4) {
+          if (g_direct_block == 512) return dense_direct_launch<K, R, GPT, 512>(psi, mat, d, ngroups, nt, s);
+          if (g_direct_block == 1024) return dense_direct_launch<K, R, GPT, 1024>(psi, mat, d, ngroups, nt, s);
+     }
      const uint64_t dblocks = (ngroups + kBlock * GPT - 1) / (kBlock * GPT);
      const dim3 grid = grid_2d(dblocks);
      // whole blocks only (a partial last block must stay last)
@@ -767,6 +794,10 @@ void set_tile_mfma(int mode) { g_tile_mfma = mode ? 1 : 0; }
 int get_tile_mfma() { return g_tile_mfma; }
 void set_dense5_tile(int on) { g_dense5_tile = on ? 1 : 0; }
 int get_dense5_tile() { return g_dense5_tile; }
+void set_direct_block(int bs) { g_direct_block = bs; }
+void set_tile_high_target_wide(int pos) { g_high_target_wide = pos; }
+int get_tile_high_target_wide() { return g_high_target_wide; }
+int get_direct_block() { return g_direct_block; }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -59,6 +59,10 @@ int get_tile_mfma();
 // one-op tile pass on the matrix cores; 0 the dense pass forms.
 void set_dense5_tile(int on);
 int get_dense5_tile();
+void set_direct_block(int bs);
+int get_direct_block();
+void set_tile_high_target_wide(int pos);
+int get_tile_high_target_wide();
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

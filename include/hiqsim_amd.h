@@ -242,6 +242,10 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *   "dense5_tile":        complex128 5-qubit passes without controls on 12+
  *                         qubits: 1 (default) as a one-op tile pass on the
  *                         matrix cores; 0 the dense pass forms.
+ *   "tile_high_target_wide": the same threshold as "tile_high_target" for
+ *                         4-qubit passes (default 13);
+ *   "direct_block":       threads per block of the direct form of 4-qubit
+ *                         passes: 256, 512 or 1024 (default).
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);
 int hiqsim_get_tuning(const char* name, int64_t* value);

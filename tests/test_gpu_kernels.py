@@ -168,6 +168,44 @@ def test_dense5_tile_route(engine_mod, oracle_mod, route):
         mod.set_tuning("dense5_tile", old)
 
 
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_direct_block_sizes(engine_mod, oracle_mod, block):
+    """4-qubit passes in the direct form (tile_high_target_wide = 6 sends
+    every 4-qubit pass with all targets at bit 6 or above there) with 256-,
+    512- and 1024-thread blocks: whole and partial grids, controls, both
+    precisions."""
+    import hiqsimulator_amd
+    mod = hiqsimulator_amd._cppsim_amd
+    old = (mod.get_tuning("direct_block"), mod.get_tuning("tile_high_target_wide"))
+    rng = np.random.default_rng(91 + block)
+    try:
+        mod.set_tuning("direct_block", block)
+        mod.set_tuning("tile_high_target_wide", 6)
+        for n in (10, 13, 18):
+            for trial in range(3):
+                pos = [int(p) for p in 6 + rng.permutation(n - 6)[:4]]
+                free = [q for q in range(n) if q not in pos]
+                cm = 0
+                for q in rng.permutation(free)[:trial]:
+                    cm |= 1 << int(q)
+                for precision in ("c128", "c64"):
+                    print("direct_block=%d n=%d pos=%s ctrl=%#x %s" % (block, n, pos, cm, precision), flush=True)
+                    st = rand_state(rng, n)
+                    m = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+                    m /= np.linalg.norm(m, 2)
+                    sim = engine_with_state(engine_mod, st, precision)
+                    sim.apply_matrix(m, pos, cm, False)
+                    ref = st.astype(np.complex64).astype(np.complex128) if precision == "c64" else st.copy()
+                    oracle_mod.dense_pass(ref, m, pos, cm)
+                    err = np.abs(sim.get_state() - ref).max() / max(np.abs(ref).max(), 1e-300)
+                    assert err <= (3e-6 if precision == "c64" else 1e-12), (n, pos, cm, precision, err)
+        with pytest.raises(RuntimeError):
+            mod.set_tuning("direct_block", 300)
+    finally:
+        mod.set_tuning("direct_block", old[0])
+        mod.set_tuning("tile_high_target_wide", old[1])
+
+
 @pytest.mark.parametrize("swizzle,high", [(0, 20), (4, 20), (8, 64), (10, 64), (16, 0)])
 def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
     """Block-order swizzle and the tile/direct threshold change only the
