@@ -12,8 +12,15 @@ amplitude per fused or permutation pass) / step wall time.
 roofline     = the dominant kernel (dense k-qubit pass): 32 * 2^30 B per
                launch / its average launch duration from HIP events recorded
                on the engine's stream over the timed region.
-cpu_baseline = the reference's own AVX2 kernels (oracle/_ref, OpenMP) on the
-               host, k-qubit passes over a 2^28-amplitude complex128 state.
+cpu_baseline = the reference's own AVX2 kernels (oracle/_ref, OpenMP with
+               OMP_PROC_BIND=spread, OMP_PLACES=cores) in a child process
+               started before this one touches the GPU: per k = 2..5 the
+               median of 5 passes on the 2^n-amplitude complex128 state, and
+               the headline circuit's fused passes end to end.
+Also reported: the step at every fusion width (circuit_k_sweep), the drop-in
+flow (run() per cluster with deferred tile passes), tile programs, the
+QFT-like precision round trip, the 4096-circuit batch and the serving
+streams (p50/p99 with 95% bootstrap intervals, served-state fidelity).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--k 4] [--qubits 30]
 """
@@ -160,39 +167,127 @@ def kernel_sweep(hq, n, reps=24):
     return out
 
 
-def cpu_baseline(k, seconds=12.0, n_sample=28):
+def cpu_info():
+    """CPU model, sockets, physical cores and the cores this process may use."""
+    model, phys = None, set()
+    pid = core = None
     try:
-        import oracle
-        from hiqsimulator_amd.circuits import haar_unitary
-    except Exception as exc:  # pragma: no cover
-        return {"value": None, "unit": "GB/s", "error": str(exc)}
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    pid = val
+                elif key == "core id":
+                    core = val
+                elif not key and pid is not None:
+                    phys.add((pid, core))
+                    pid = core = None
+        if pid is not None:
+            phys.add((pid, core))
+    except OSError:
+        pass
+    return {"model": model, "sockets": len({p for p, _ in phys}) or None,
+            "physical_cores": len(phys) or None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline_child(args):
+    """The reference CPU path, run in a child process whose OpenMP runtime
+    starts with OMP_PROC_BIND=spread and OMP_PLACES=cores (the reference's
+    advice, _simulator_mpi.py:52-59): the reference's own AVX2 kernels
+    (oracle/_ref, built with its release flags) dispatched as its Run() does.
+    * per k = 2..5: one warm pass, then the median of --cpu-reps passes at
+      random placements on the 2^n-amplitude complex128 state;
+    * the whole headline circuit: every fused pass of the step (the
+      reference's fusion, then one kernel pass per cluster); the step's
+      qubit permutations have no counterpart on the reference's one-node path."""
+    import oracle
+    from hiqsimulator_amd.circuits import haar_unitary
+    n = args.qubits
+    if not oracle.RefKernels.available("intrin"):
+        return {"value": None, "unit": "GB/s", "error": "oracle/_ref not built"}
+    ref = oracle.RefKernels("intrin")
+    t0 = time.perf_counter()
+    h, _ = ref.new_state(1 << n)
+    ref.fill(h, 2.0 ** (-n / 2))
+    alloc_s = time.perf_counter() - t0
     rng = np.random.default_rng(3)
-    if oracle.RefKernels.available("intrin"):
-        ref = oracle.RefKernels("intrin")
-        kind, cores = "reference", ref.num_threads()
-        h, arr = ref.new_state(1 << n_sample)
-        arr.fill(2.0 ** (-n_sample / 2))
-        apply = lambda m, pos: ref.apply(h, m, pos)  # noqa: E731
-    else:
-        kind, cores, n_sample = "port", 1, 22
-        arr = np.full(1 << n_sample, 2.0 ** (-n_sample / 2), dtype=np.complex128)
-        h = None
-        apply = lambda m, pos: oracle.dense_pass(arr, m, pos)  # noqa: E731
-    passes, t0 = 0, time.perf_counter()
-    while True:
-        m = haar_unitary(1 << k, rng)
-        pos = [int(p) for p in rng.permutation(n_sample)[:k]]
-        apply(m, pos)
-        passes += 1
+    per_k = {}
+    for k in range(2, 6):
+        times = []
+        for rep in range(args.cpu_reps + 1):
+            m = haar_unitary(1 << k, rng)
+            pos = [int(p) for p in rng.permutation(n)[:k]]
+            t0 = time.perf_counter()
+            ref.apply(h, m, pos)
+            if rep:
+                times.append(time.perf_counter() - t0)
+        med = float(np.median(times))
+        per_k[str(k)] = {"median_s": round(med, 4), "gbs": round(32.0 * 2 ** n / med / 1e9, 2),
+                         "passes": len(times)}
+        print("cpu k=%d: %s" % (k, per_k[str(k)]), file=sys.stderr, flush=True)
+    circuit = None
+    if not args.no_cpu_circuit:
+        t0 = time.perf_counter()
+        gates, step, _ = build_step(n, args.layers, args.k, args.seed, args.perms, args.planner)
+        plan_s = time.perf_counter() - t0
+        passes = [payload for kind, payload in step if kind == "pass"]
+        ref.fill(h, 2.0 ** (-n / 2))
+        t0 = time.perf_counter()
+        for cl in passes:
+            f = oracle.OracleFusion()
+            for m, t, c in cl:
+                mm = np.asarray(m, dtype=np.complex128)
+                if c:  # controls folded as the highest matrix bits (fusion_mpi.hpp:167-187)
+                    d = mm.shape[0]
+                    full = np.eye(d << len(c), dtype=np.complex128)
+                    full[-d:, -d:] = mm
+                    mm = full
+                f.insert(mm, list(t) + list(c))
+            M, ids, _, diag = f.perform(cap_qubits=5)
+            ref.apply(h, M, ids, 0, diag)
         el = time.perf_counter() - t0
-        if (el > seconds and passes >= 3) or passes >= 400:
-            break
-    if h is not None:
-        ref.free_state(h)
-    gbs = 32.0 * 2 ** n_sample * passes / el / 1e9
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores, "kind": kind,
-            "sample": "%d dense k=%d passes over a 2^%d-amplitude complex128 state, %.1f s"
-                      % (passes, k, n_sample, el)}
+        circuit = {"seconds": round(el, 2), "fused_passes": len(passes), "gates": len(gates),
+                   "fusion_k": args.k, "plan_seconds": round(plan_s, 2),
+                   "gbs": round(32.0 * 2 ** n * len(passes) / el / 1e9, 2),
+                   "note": "the headline step's fused passes (reference fusion + kernels); its %d qubit "
+                           "permutations have no counterpart on the reference's one-node path"
+                           % sum(1 for kind, _ in step if kind == "perm")}
+        print("cpu circuit: %s" % circuit, file=sys.stderr, flush=True)
+    ref.free_state(h)
+    info = cpu_info()
+    return {"value": per_k[str(args.k)]["gbs"], "unit": "GB/s", "cores": ref.num_threads(),
+            "kind": "reference",
+            "sample": "reference AVX2 kernels (oracle/_ref, -O3 -ffast-math -mavx2 -mfma, OpenMP) on a "
+                      "2^%d-amplitude complex128 state: per k = 2..5 the median of %d passes at random "
+                      "placements; value = k=%d" % (n, args.cpu_reps, args.k),
+            "per_k": per_k, "circuit": circuit, "cpu": info,
+            "omp": {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "state_alloc_s": round(alloc_s, 2)}
+
+
+def cpu_baseline(args):
+    """Runs cpu_baseline_child in a fresh Python process (before this process
+    touches the GPU), so that OpenMP starts with the thread binding set."""
+    import subprocess
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "spread"
+    env["OMP_PLACES"] = "cores"
+    if "OMP_NUM_THREADS" not in env:
+        phys = cpu_info()["physical_cores"] or os.cpu_count() or 1
+        env["OMP_NUM_THREADS"] = str(min(phys, len(os.sched_getaffinity(0))))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--qubits", str(args.qubits),
+           "--layers", str(args.layers), "--k", str(args.k), "--seed", str(args.seed), "--perms",
+           str(args.perms), "--planner", args.planner, "--cpu-reps", str(args.cpu_reps)]
+    if args.no_cpu_circuit:
+        cmd.append("--no-cpu-circuit")
+    r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        return {"value": None, "unit": "GB/s", "error": "cpu baseline exited with %d" % r.returncode}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def bench_precision(hq, n, log_fn):
@@ -269,18 +364,64 @@ def bench_batch(hq, count, seed, log_fn, plain_sample=64):
     errs = [float(np.abs(a - b).max()) for a, b in zip(states["c128"], states["c64"])]
     out["c64_vs_c128"] = {"sample": len(fids), "min_fidelity": min(fids), "max_abs": max(errs)}
     sample = [circuits[i] for i in sample_idx[:plain_sample]]
-    plain_s = one_at_a_time(sample, "c64")
-    out["plain_one_at_a_time_c64"] = {"sample": len(sample),
-                                      "circuits_per_s": round(len(sample) / plain_s, 1)}
-    log_fn("batch: plain path %s" % out["plain_one_at_a_time_c64"])
+    for mode in ("fused", "deferred", "tiled"):
+        plain_s = one_at_a_time(sample, "c64", mode=mode)
+        out["plain_one_at_a_time_c64_" + mode] = {"sample": len(sample),
+                                                 "circuits_per_s": round(len(sample) / plain_s, 1)}
+        log_fn("batch: plain path (%s) %s" % (mode, out["plain_one_at_a_time_c64_" + mode]))
+    best = max(out["plain_one_at_a_time_c64_" + m]["circuits_per_s"] for m in ("fused", "deferred", "tiled"))
+    out["c64_vs_best_plain"] = round(out["c64"]["circuits_per_s"] / best, 2)
     return out
 
 
-def bench_serving(count, rates, seed, log_fn):
+def bootstrap_ci(lat_ms, q, reps=2000, seed=0):
+    """95% bootstrap interval of the q-th percentile of lat_ms."""
+    lat = np.asarray(lat_ms)
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, lat.size, size=(reps, lat.size))
+    vals = np.percentile(lat[idx], q, axis=1)
+    return [round(float(np.percentile(vals, 2.5)), 2), round(float(np.percentile(vals, 97.5)), 2)]
+
+
+def served_fidelity(hq, reqs, log_fn, count=8, max_qubits=22):
+    """A sample of a stream's circuits served again as state requests (in
+    one mix, complex64 tile programs) against the complex128 engine on the
+    same circuit (one fused pass per cluster): fidelity and max-abs."""
+    from hiqsimulator_amd.scheduling import plan_clusters
+    from hiqsimulator_amd.serving import make_server, result, submit
+    pick = [r for r in reqs if r["n"] <= max_qubits][:count]
+    s = make_server("c64", state_bytes=8 << 30, matrix_bytes=1 << 30)
+    ids = [submit(s, dict(r, kind="state")) for r in pick]
+    while not s.idle():
+        s.step(block=True)
+    fids, errs = [], []
+    for rid, r in zip(ids, pick):
+        got = result(s, rid, dict(r, kind="state"))
+        n = r["n"]
+        sim = hq.SimulatorMPI(1, n, 4, "c128")
+        sim.set_pass_mode(0)
+        sim.allocate_qureg(list(range(n)), 0)
+        for cl in plan_clusters(r["gates"], list(range(n)), 4, mode="lookahead"):
+            for j in cl:
+                sim.apply_controlled_gate(*r["gates"][j])
+            sim.run()
+        ref = sim.get_state()
+        del sim
+        fids.append(float(abs(np.vdot(ref, got)) ** 2 / (np.vdot(ref, ref).real * np.vdot(got, got).real)))
+        errs.append(float(np.abs(ref - got).max()))
+    out = {"sample": len(pick), "qubits": [r["n"] for r in pick], "min_fidelity": min(fids),
+           "one_minus_fidelity_max": max(0.0, 1.0 - min(fids)), "max_abs": max(errs),
+           "reference": "complex128 engine, one fused pass per cluster", "target": "fidelity >= 1 - 1e-6"}
+    log_fn("served fidelity: %s" % out)
+    return out
+
+
+def bench_serving(hq, count, rates, seed, log_fn):
     """Poisson request streams, n log-uniform in 12..28, depth ~ U{20..100},
     1024 samples or 8 Pauli expectations each, complex64; the request loop
     (tile programs from 12 qubits) against one-circuit-at-a-time
-    SimulatorMPI (apply_tiled) on the same stream."""
+    SimulatorMPI (apply_tiled) on the same stream.  p50/p99 carry 95%
+    bootstrap intervals."""
     from hiqsimulator_amd.serving import make_server, run_plain, run_server, serving_workload
     out = []
     for rate in rates:
@@ -289,13 +430,113 @@ def bench_serving(count, rates, seed, log_fn):
         srv, _ = run_server(server, reqs)
         st = server.stats()
         del server
-        log_fn("serving rate %g: server %s" % (rate, srv))
+        log_fn("serving rate %g: server %s" % (rate, {k: v for k, v in srv.items() if k != "lat_ms"}))
         plain, _ = run_plain(reqs, "c64", tiled=True)
-        log_fn("serving rate %g: plain %s" % (rate, plain))
-        out.append({"rate_per_s": rate, "requests": count,
-                    "server": {k: round(v, 3) for k, v in srv.items()},
-                    "plain": {k: round(v, 3) for k, v in plain.items()},
-                    "peak_active": st["peak_active"], "steps": st["steps"]})
+        log_fn("serving rate %g: plain %s" % (rate, {k: v for k, v in plain.items() if k != "lat_ms"}))
+        entry = {"rate_per_s": rate, "requests": count, "peak_active": st["peak_active"], "steps": st["steps"]}
+        for name, res in (("server", srv), ("plain", plain)):
+            lat = res.pop("lat_ms")
+            entry[name] = {k: round(v, 3) for k, v in res.items()}
+            entry[name]["p50_ci95_ms"] = bootstrap_ci(lat, 50)
+            entry[name]["p99_ci95_ms"] = bootstrap_ci(lat, 99)
+        out.append(entry)
+    fid = served_fidelity(hq, serving_workload(count, rates[0], seed), log_fn)
+    return {"streams": out, "fidelity": fid}
+
+
+def bench_drop_in(hq, n, layers, k, seed, steps, log_fn):
+    """The drop-in flow: the headline circuit through the GreedyScheduler
+    loop (clusters planned beforehand), apply_controlled_gate per gate and
+    run() per cluster, as the reference's Python engine drives its C++
+    object.  In the default (deferred) mode the clusters run as tile passes;
+    with pass mode 0 each is one fused pass, as in the reference."""
+    from hiqsimulator_amd.circuits import random_layers_circuit
+    from hiqsimulator_amd.scheduling import plan_clusters
+    gates = random_layers_circuit(n, layers, seed)
+    clusters = plan_clusters(gates, list(range(n)), k)
+    out = {"workload": "%dq %d-layer circuit (%d gates), GreedyScheduler loop (%d clusters), "
+                       "apply_controlled_gate + run() per cluster" % (n, layers, len(gates), len(clusters))}
+    sim = hq.SimulatorMPI(seed, n, k)
+    sim.allocate_qureg(list(range(n)), 0)
+
+    def feed():
+        for cl in clusters:
+            for i in cl:
+                m, t, c = gates[i]
+                sim.apply_controlled_gate(m, t, c)
+            sim.run()
+        sim.synchronize()
+
+    for mode, name in ((1, "deferred"), (0, "eager")):
+        sim.set_pass_mode(mode)
+        feed()  # warm
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            feed()
+        el = (time.perf_counter() - t0) / steps
+        tiles = sim.kernel_time(hq.KERNEL_CLASSES.index("tile"))[0] // steps
+        dense = sum(sim.kernel_time(c)[0] for c in range(6)) // steps
+        sim.set_timing(False)
+        out[name] = {"ms_per_circuit": round(el * 1e3, 2), "hbm_passes_per_circuit": int(tiles + dense)}
+    out["drop_in_ms_per_circuit"] = out["deferred"]["ms_per_circuit"]
+    out["speedup_vs_eager"] = round(out["eager"]["ms_per_circuit"] / out["deferred"]["ms_per_circuit"], 3)
+    # the deferred result equals the eager one: the circuit, then its inverse
+    # eagerly, returns a random state to itself
+    inv = [(m.conj().T, t, c) for (m, t, c) in reversed(gates)]
+    sim.init_random_state(5)
+    sim.snapshot()
+    sim.set_pass_mode(1)
+    for cl in clusters:
+        for i in cl:
+            sim.apply_controlled_gate(*gates[i])
+        sim.run()
+    sim.set_pass_mode(0)
+    for cl in reversed(clusters):
+        for i in reversed(cl):
+            sim.apply_controlled_gate(*inv[len(gates) - 1 - i])
+        sim.run()
+    r = sim.compare_snapshot()
+    out["deferred_then_eager_inverse_max_abs"] = float(r["max_abs_diff"])
+    del sim
+    log_fn("drop-in: %s" % out)
+    return out
+
+
+def bench_circuit_k_sweep(hq, n, layers, seed, perms, planner, ks, log_fn):
+    """The headline step at every fusion width k: ms per step, fused passes,
+    average pass time (HIP events) and GB/s, and the error of one step and
+    its exact inverse (compared on the device in FP64)."""
+    out = {}
+    sim = hq.SimulatorMPI(seed, n, 5)
+    sim.set_pass_mode(0)
+    sim.allocate_qureg(list(range(n)), 0)
+    for k in ks:
+        gates, step, _ = build_step(n, layers, k, seed, perms, planner)
+        n_pass = sum(1 for kind, _ in step if kind == "pass")
+        sim.set_qubits_perm(list(range(n)))
+        sim.init_random_state(k)
+        run_step(sim, step)  # warm
+        sim.synchronize()
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        run_step(sim, step)
+        sim.synchronize()
+        el = time.perf_counter() - t0
+        cnt = sum(sim.kernel_time(c)[0] for c in range(6))
+        ms = sum(sim.kernel_time(c)[1] for c in range(6))
+        sim.set_timing(False)
+        start = list(sim.get_qubits_ids())
+        sim.snapshot()
+        run_step(sim, step)
+        run_step(sim, inverse_step(step, start))
+        r = sim.compare_snapshot()
+        out[str(k)] = {"ms_per_step": round(el * 1e3, 1), "fused_passes": n_pass,
+                       "avg_pass_ms": round(ms / cnt, 4), "pass_gbs": round(32.0 * 2 ** n / (ms / cnt * 1e-3) / 1e9, 1),
+                       "step_gbs": round(32.0 * 2 ** n * (n_pass + perms) / el / 1e9, 1),
+                       "round_trip_max_abs": float(r["max_abs_diff"])}
+        log_fn("circuit k=%d: %s" % (k, out[str(k)]))
+    del sim
     return out
 
 
@@ -324,6 +565,15 @@ def main():
     ap.add_argument("--planner", choices=["greedy", "inorder"], default="greedy",
                     help="greedy: the GreedyScheduler/ClusterScheduler loop; inorder: program order")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="run only the CPU baseline and print its JSON (the child process bench.py starts)")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU passes per k (median)")
+    ap.add_argument("--no-cpu-circuit", action="store_true",
+                    help="skip the whole-circuit CPU timing (about a minute at 30 qubits)")
+    ap.add_argument("--no-drop-in", action="store_true")
+    ap.add_argument("--no-circuit-sweep", action="store_true")
+    ap.add_argument("--circuit-ks", default="2,3,4,5",
+                    help="fusion widths of the whole-step sweep")
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the step-and-inverse error check of the headline circuit")
     ap.add_argument("--no-sweep", action="store_true")
@@ -335,7 +585,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rates", default="20,60,120")
     ap.add_argument("--block-swizzle", type=int, default=None,
                     help="low block-index bits reversed in the streaming passes (library default 8)")
@@ -351,10 +601,20 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain state copy (snapshot) for counter calibration")
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        print(json.dumps(cpu_baseline_child(args)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    # The CPU baseline runs first, in a child process, before this process
+    # touches the GPU (its OpenMP runtime starts with the thread binding set).
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args)
+        log("cpu baseline: %s" % cpu)
 
     import hiqsimulator_amd as hq
     if args.block_swizzle is not None:
@@ -380,6 +640,7 @@ def main():
         % (len(gates), n_pass, n_perm, plan_s))
 
     sim = hq.SimulatorMPI(args.seed, n, k)
+    sim.set_pass_mode(0)  # the reference's one fused pass per run()
     sim.allocate_qureg(list(range(n)), 0)
     sim.synchronize()
     if args.calibrate_copy:
@@ -477,15 +738,17 @@ def main():
     sweep = None if args.no_sweep else kernel_sweep(hq, n)
     if sweep:
         log("per-pass GB/s by k: %s" % sweep)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(k)
-    if cpu:
-        log("cpu baseline: %s" % cpu)
+    drop_in = None if args.no_drop_in else bench_drop_in(hq, n, args.layers, k, args.seed, args.steps, log)
+    circuit_sweep = None
+    if not args.no_circuit_sweep:
+        circuit_sweep = bench_circuit_k_sweep(hq, n, args.layers, args.seed, args.perms, args.planner,
+                                              [int(x) for x in args.circuit_ks.split(",") if x], log)
     precision = None if args.no_precision else bench_precision(hq, args.precision_qubits, log)
     batch = None if args.no_batch else bench_batch(hq, args.batch_circuits, args.seed, log)
     serving = None
     if not args.no_serving:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
-        serving = bench_serving(args.serve_requests, rates, args.seed, log)
+        serving = bench_serving(hq, args.serve_requests, rates, args.seed, log)
 
     line = {
         "metric": "30q fp64 fused-pass HBM GB/s vs peak; mixed-size circuits/s + p99 latency",
@@ -516,7 +779,9 @@ def main():
                   "gates_per_s": round(len(gates) / (ms_step * 1e-3), 1),
                   "permutations_per_s": round(n_perm / (ms_step * 1e-3), 2)},
         "k_sweep_gbs": sweep,
+        "circuit_k_sweep": circuit_sweep,
         "tile_planner": tile,
+        "drop_in": drop_in,
         "cpu_baseline": cpu,
         "parity": parity,
         "precision": precision,

@@ -171,22 +171,33 @@ def autotune(ns, precision="c128", widths=(2, 3, 4, 5), depth=20, amp_budget=1 <
             "probe_ms": {"%d/%s/k%d" % key: round(v, 4) for key, v in probe.items()}}
 
 
-def one_at_a_time(circuits, precision="c128", k=4, device=0):
-    """The plain path: one SimulatorMPI per circuit, gates fed through
-    apply_controlled_gate with run() per fused cluster of at most k qubits
-    (lookahead plan), the final state synchronised.  Returns seconds."""
+def one_at_a_time(circuits, precision="c128", k=4, device=0, mode="fused"):
+    """The plain path: one SimulatorMPI per circuit, the final state
+    synchronised.  Returns seconds.  ``mode``:
+    * "fused": apply_controlled_gate with run() per fused cluster of at most
+      k qubits (lookahead plan), one HBM pass per cluster (pass mode 0);
+    * "deferred": the same calls in the default pass mode (clusters of
+      12+-qubit states run as tile passes);
+    * "tiled": the whole gate list through apply_tiled (op width k)."""
     from . import SimulatorMPI
     from .scheduling import plan_clusters
+    if mode not in ("fused", "deferred", "tiled"):
+        raise ValueError("mode must be 'fused', 'deferred' or 'tiled'")
     plans = [plan_clusters(g, list(range(n)), k, mode="lookahead") for n, g in circuits]
     t0 = time.perf_counter()
     for (n, gates), clusters in zip(circuits, plans):
         sim = SimulatorMPI(1, n, k, precision, device)
+        if mode == "fused":
+            sim.set_pass_mode(0)
         sim.allocate_qureg(list(range(n)), 0)
-        for cl in clusters:
-            for i in cl:
-                m, t, c = gates[i]
-                sim.apply_controlled_gate(m, t, c)
-            sim.run()
+        if mode == "tiled":
+            sim.apply_tiled(gates, k)
+        else:
+            for cl in clusters:
+                for i in cl:
+                    m, t, c = gates[i]
+                    sim.apply_controlled_gate(m, t, c)
+                sim.run()
         sim.synchronize()
         del sim
     return time.perf_counter() - t0

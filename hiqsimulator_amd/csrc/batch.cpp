@@ -437,7 +437,7 @@ Batch::HostPlan Batch::plan_host(int workgroups)
                     const PlannedPass& pp = plan[c][L];
                     const int n = circ_[c].n;
                     BatchTileItem ti{};
-                    const bool tile_pays = n >= tile_min_qubits_ && pp.pos[0] < get_tile_high_target() &&
+                    const bool tile_pays = n >= tile_min_qubits_ && pp.pos[0] < get_tile_high_target_for(k) &&
                                            (k < 5 || pp.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
                     if (tile_pays && make_tile_item(n, k, pp.pos, prec_, ti)) {
                          ti.state_off = circ_[c].state_off;

@@ -140,7 +140,7 @@ hq::GateList gate_list(const int64_t* gate_ids, const size_t* gate_off, const in
 
 extern "C" {
 
-int hiqsim_version(void) { return 1; }
+int hiqsim_version(void) { return 2; }
 
 const char* hiqsim_last_error(void) { return g_err.c_str(); }
 
@@ -331,6 +331,45 @@ int hiqsim_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precisio
           *n_amps = h->eng.state_size();
           if (precision) *precision = static_cast<int>(h->eng.precision());
           if (device) *device = h->eng.device();
+     });
+}
+
+int hiqsim_pin_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precision, int* device)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(ptr, "ptr");
+          need(n_amps, "n_amps");
+          *ptr = h->eng.pin_state();
+          *n_amps = h->eng.state_size();
+          if (precision) *precision = static_cast<int>(h->eng.precision());
+          if (device) *device = h->eng.device();
+     });
+}
+
+int hiqsim_unpin_device_state(hiqsim_t* h)
+{
+     return guard([&] {
+          need(h, "handle");
+          h->eng.unpin_state();
+     });
+}
+
+int hiqsim_set_pass_mode(hiqsim_t* h, int mode)
+{
+     return guard([&] {
+          need(h, "handle");
+          h->eng.set_pass_mode(mode);
+     });
+}
+
+int hiqsim_get_pass_mode(hiqsim_t* h, int* mode, size_t* queued)
+{
+     return guard([&] {
+          need(h, "handle");
+          need(mode, "mode");
+          *mode = h->eng.pass_mode();
+          if (queued) *queued = h->eng.queued_clusters();
      });
 }
 
@@ -802,7 +841,8 @@ int hiqsim_server_submit(hiqsim_server_t* s, int n_qubits, size_t ngates, const 
      return guard([&] {
           need(s, "server");
           need(id, "id");
-          if (kind != 0 && kind != 1) throw std::runtime_error("kind must be 0 (samples) or 1 (expectation)");
+          if (kind < 0 || kind > 2)
+               throw std::runtime_error("kind must be 0 (samples), 1 (expectation) or 2 (state)");
           std::vector<uint64_t> X, Z;
           if (kind == 1 && nterms) {
                need(xmask, "xmask");
@@ -868,6 +908,20 @@ int hiqsim_server_expectations(hiqsim_server_t* s, int64_t id, double* out, size
                out[2 * i + 1] = v[i].imag();
           }
           if (n) *n = v.size();
+     });
+}
+
+int hiqsim_server_state(hiqsim_server_t* s, int64_t id, double* out, uint64_t cap_amps, uint64_t* n_amps)
+{
+     return guard([&] {
+          need(s, "server");
+          const auto& v = s->s.state(id);
+          if (cap_amps) need(out, "out");
+          for (uint64_t i = 0; i < v.size() && i < cap_amps; ++i) {
+               out[2 * i] = v[i].real();
+               out[2 * i + 1] = v[i].imag();
+          }
+          if (n_amps) *n_amps = v.size();
      });
 }
 

@@ -97,6 +97,7 @@ Engine::~Engine()
           if (ring_ev_[i]) (void)hipEventDestroy(ring_ev_[i]);
      if (d_state_) (void)hipFree(d_state_);
      if (d_tmp_) (void)hipFree(d_tmp_);
+     for (void* p: retired_) (void)hipFree(p);
      if (d_snap_) (void)hipFree(d_snap_);
      if (d_ring_) (void)hipFree(d_ring_);
      if (h_ring_) (void)hipHostFree(h_ring_);
@@ -106,27 +107,49 @@ Engine::~Engine()
      if (h_blocks_) (void)hipHostFree(h_blocks_);
      if (h_stage_) (void)hipHostFree(h_stage_);
      if (d_samples_) (void)hipFree(d_samples_);
-     if (prog_ev_) {
-          (void)hipEventSynchronize(prog_ev_);
-          (void)hipEventDestroy(prog_ev_);
+     for (auto& ps: prog_) {
+          if (ps.ev) (void)hipEventDestroy(ps.ev);
+          if (ps.h) (void)hipHostFree(ps.h);
+          if (ps.d) (void)hipFree(ps.d);
      }
-     if (h_prog_) (void)hipHostFree(h_prog_);
-     if (d_prog_) (void)hipFree(d_prog_);
      if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 // ---------------------------------------------------------------------------
 // Buffers
 // ---------------------------------------------------------------------------
+// A state buffer that may be viewed is retired instead of freed while a pin
+// is held; collect_retired() frees the retired buffers once no pin is left.
+void Engine::release_buffer(void*& p)
+{
+     if (!p) return;
+     if (view_pins_.load() > 0) {
+          retired_.push_back(p);
+     }
+     else {
+          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+          check(hipFree(p), "hipFree");
+     }
+     p = nullptr;
+}
+
+void Engine::collect_retired()
+{
+     if (retired_.empty() || view_pins_.load() > 0) return;
+     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+     for (void* p: retired_) check(hipFree(p), "hipFree");
+     retired_.clear();
+}
+
 void Engine::ensure_state(uint64_t n)
 {
+     collect_retired();
      if (n <= cap_state_) return;
      void* p = nullptr;
      check(hipMalloc(&p, n * amp_bytes(prec_)), "hipMalloc(state)");
      if (d_state_) {
           check(launch_copy(p, d_state_, prec_, n_amps_, stream_), "copy");
-          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-          check(hipFree(d_state_), "hipFree");
+          release_buffer(d_state_);
      }
      d_state_ = p;
      cap_state_ = n;
@@ -134,13 +157,10 @@ void Engine::ensure_state(uint64_t n)
 
 void Engine::ensure_tmp(uint64_t n)
 {
+     collect_retired();
      if (n <= cap_tmp_) return;
-     if (d_tmp_) {
-          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-          check(hipFree(d_tmp_), "hipFree");
-          d_tmp_ = nullptr;
-          cap_tmp_ = 0;
-     }
+     release_buffer(d_tmp_);
+     cap_tmp_ = 0;
      check(hipMalloc(&d_tmp_, n * amp_bytes(prec_)), "hipMalloc(tmp)");
      cap_tmp_ = n;
 }
@@ -149,6 +169,18 @@ void Engine::swap_tmp()
 {
      std::swap(d_state_, d_tmp_);
      std::swap(cap_state_, cap_tmp_);
+}
+
+void Engine::commit_tmp()
+{
+     if (view_pins_.load() > 0) {
+          begin_timed(kFillCopy);
+          check(launch_copy(d_state_, d_tmp_, prec_, n_amps_, stream_), "copy");
+          end_timed();
+     }
+     else {
+          swap_tmp();
+     }
 }
 
 const void* Engine::upload_matrix(const Mat& m, const void* desc, size_t desc_bytes)
@@ -241,6 +273,7 @@ void Engine::kernel_time(int cls, uint64_t& count, double& total_ms)
 {
      DeviceScope ds(device_);
      if (cls < 0 || cls >= kNumKernelClasses) throw std::runtime_error("kernel_time(): bad class");
+     settle();
      check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
      for (auto& p: pending_) {
           float ms = 0;
@@ -258,6 +291,7 @@ void Engine::kernel_time(int cls, uint64_t& count, double& total_ms)
 void Engine::synchronize()
 {
      DeviceScope ds(device_);
+     settle();
      check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
@@ -292,6 +326,7 @@ void Engine::allocate_qureg(const std::vector<int64_t>& ids, cplx init)
      const size_t limit = std::max(min_local_, max_local_);
      const size_t room = locals_.size() < limit ? limit - locals_.size() : 0;
      const size_t count = std::min(room, ids.size());
+     settle();
      if (count > 0) {
           if (locals_.size() + count > 62)
                throw std::runtime_error("AllocateQubit(): state vector too large");
@@ -319,6 +354,7 @@ void Engine::deallocate_qubit(int64_t id)
 {
      DeviceScope ds(device_);
      const size_t pos = find_sure(id);
+     settle();
      begin_timed(kReduce);
      check(launch_two_bucket(d_state_, prec_, n_qubits(), static_cast<int>(pos), d_scratch_,
                              d_scratch_ + kReduceScratchDoubles, stream_),
@@ -376,11 +412,12 @@ void Engine::permute_qubits(const std::vector<int64_t>& ids)
      if (seen != (ids.empty() ? 0 : ((uint64_t(1) << ids.size()) - 1)))
           throw std::runtime_error("permute_qubits(): ids must be a permutation of the qubits");
      if (identity) return;
+     settle();
      ensure_tmp(n_amps_);
      begin_timed(kPermute);
      check(launch_permute(d_tmp_, d_state_, prec_, n_qubits(), src.data(), stream_), "permute");
      end_timed();
-     swap_tmp();
+     commit_tmp();
      locals_ = ids;
 }
 
@@ -428,23 +465,54 @@ void Engine::run()
      }
      std::vector<uint32_t> pos(ids.size());
      for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
-     if (ids.empty()) {
-          if (m(0, 0) != cplx(1.0, 0.0)) {
-               begin_timed(kScale);
-               check(launch_scale(d_state_, prec_, n_amps_, m(0, 0).real(), m(0, 0).imag(),
-                                  stream_),
-                     "scale");
-               end_timed();
-          }
-     }
-     else if (ids.size() <= 5) {
-          dense_or_diag(m, pos, cmask, diag);
-     }
-     else {
+     if (ids.size() > 5)
           throw std::runtime_error("Run(): cannot apply " + std::to_string(ids.size())
                                    + " qubits gate");
+     if (ids.empty() && m(0, 0) == cplx(1.0, 0.0)) {
+          fusion_ = Fusion();
+          return;
+     }
+     const size_t nctrl = static_cast<size_t>(__builtin_popcountll(cmask));
+     if (pass_mode_ == kPassDeferred && !ids.empty() && n_qubits() >= kTileBits && ids.size() + nctrl <= 5) {
+          PosGate g;
+          g.m = std::move(m);
+          g.t.assign(pos.begin(), pos.end());
+          for (int p = 0; p < 64; ++p)
+               if ((cmask >> p) & 1) g.c.push_back(p);
+          queue_.push_back(std::move(g));
+          fusion_ = Fusion();
+          if (queue_.size() >= kQueueHigh) drain_queue(kQueueLow);
+          return;
+     }
+     settle();
+     if (ids.empty()) {
+          begin_timed(kScale);
+          check(launch_scale(d_state_, prec_, n_amps_, m(0, 0).real(), m(0, 0).imag(), stream_), "scale");
+          end_timed();
+     }
+     else {
+          dense_or_diag(m, pos, cmask, diag);
      }
      fusion_ = Fusion();
+}
+
+void Engine::set_pass_mode(int mode)
+{
+     if (mode != kPassEach && mode != kPassDeferred)
+          throw std::runtime_error("set_pass_mode(): mode must be 0 (one pass per run) or 1 (deferred tile passes)");
+     if (mode != pass_mode_) {
+          DeviceScope ds(device_);
+          settle();
+     }
+     pass_mode_ = mode;
+}
+
+void Engine::drain_queue(size_t keep)
+{
+     if (queue_.size() <= keep || queue_.empty()) return;
+     int width = 4;
+     for (const auto& g: queue_) width = std::max(width, static_cast<int>(g.t.size() + g.c.size()));
+     tile_items(queue_, width, keep);
 }
 
 void Engine::dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
@@ -551,6 +619,7 @@ void Engine::apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64
      }
      if ((ctrlmask & tm) || (locals_.size() < 64 && (ctrlmask >> locals_.size())))
           throw std::runtime_error("apply_matrix(): bad control mask");
+     settle();
      dense_or_diag(m, pos, ctrlmask, diag && is_diagonal(m));
 }
 
@@ -590,6 +659,7 @@ void Engine::normalize(double norm, uint64_t msk, uint64_t val)
 std::vector<uint8_t> Engine::measure(const std::vector<int64_t>& ids)
 {
      DeviceScope ds(device_);
+     settle();
      const uint64_t nb = std::min<uint64_t>(n_amps_, kMaxBlocks);
      const uint64_t K = n_amps_ / nb;
      begin_timed(kReduce);
@@ -661,6 +731,7 @@ double Engine::get_probability(const std::vector<uint8_t>& bits, const std::vect
           throw std::runtime_error("GetProbability(): ids.size() != bit_string.size()");
      uint64_t msk, val;
      mask_of(ids, bits, msk, val);
+     settle();
      return masked_norm(msk, val);
 }
 
@@ -680,6 +751,7 @@ cplx Engine::get_amplitude(const std::vector<uint8_t>& bits, const std::vector<i
           throw std::runtime_error(
               "GetAmplitude(): the second argument must be a permutation of all allocated "
               "qubits.");
+     settle();
      const size_t ab = amp_bytes(prec_);
      char buf[16];
      check(hipMemcpyAsync(buf, static_cast<char*>(d_state_) + num * ab, ab, hipMemcpyDeviceToHost,
@@ -703,6 +775,7 @@ void Engine::collapse(const std::vector<int64_t>& ids, const std::vector<uint8_t
           throw std::runtime_error("collapseWaveFunction(): ids.size() != values.size()");
      uint64_t msk, val;
      mask_of(ids, values, msk, val);
+     settle();
      const double norm = masked_norm(msk, val);
      if (norm < 1.e-12)
           throw std::runtime_error("collapseWaveFunction(): Invalid collapse! Probability is ~0.");
@@ -712,6 +785,7 @@ void Engine::collapse(const std::vector<int64_t>& ids, const std::vector<uint8_t
 double Engine::entropy()
 {
      DeviceScope ds(device_);
+     settle();
      begin_timed(kReduce);
      check(launch_entropy(d_state_, prec_, n_amps_, d_scratch_, d_scratch_ + kReduceScratchDoubles,
                           stream_),
@@ -728,8 +802,23 @@ void* Engine::device_state()
 {
      DeviceScope ds(device_);
      run();
+     settle();
      check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
      return d_state_;
+}
+
+void* Engine::pin_state()
+{
+     void* p = device_state();
+     view_pins_.fetch_add(1);
+     return p;
+}
+
+void Engine::unpin_state()
+{
+     int v = view_pins_.load();
+     while (v > 0 && !view_pins_.compare_exchange_weak(v, v - 1)) {
+     }
 }
 
 namespace {
@@ -771,6 +860,7 @@ void Engine::save_state(const std::string& path)
 {
      DeviceScope ds(device_);
      run();
+     settle();
      File f(path, "wb");
      const uint32_t hdr[3] = {kStateVersion, static_cast<uint32_t>(prec_), static_cast<uint32_t>(locals_.size())};
      f.write(kStateMagic, 4);
@@ -791,6 +881,7 @@ void Engine::load_state(const std::string& path)
 {
      DeviceScope ds(device_);
      run();
+     settle();
      File f(path, "rb");
      char magic[4];
      uint32_t hdr[3];
@@ -853,6 +944,7 @@ void Engine::load_state(const std::string& path)
 void Engine::get_state_c128(std::complex<double>* out)
 {
      DeviceScope ds(device_);
+     settle();
      if (prec_ == Prec::C128) {
           check(hipMemcpyAsync(out, d_state_, n_amps_ * 16, hipMemcpyDeviceToHost, stream_),
                 "hipMemcpyAsync(state)");
@@ -872,6 +964,7 @@ void Engine::set_state_c128(const std::complex<double>* in, uint64_t n)
      if (n != n_amps_)
           throw std::runtime_error("set_wavefunction(): expected " + std::to_string(n_amps_)
                                    + " amplitudes, got " + std::to_string(n));
+     settle();
      if (prec_ == Prec::C128) {
           check(hipMemcpyAsync(d_state_, in, n * 16, hipMemcpyHostToDevice, stream_),
                 "hipMemcpyAsync(state)");
@@ -891,6 +984,7 @@ void Engine::set_state_c128(const std::complex<double>* in, uint64_t n)
 void Engine::snapshot()
 {
      DeviceScope ds(device_);
+     settle();
      if (n_amps_ > cap_snap_) {
           if (d_snap_) {
                check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -913,10 +1007,10 @@ void Engine::restore()
 {
      DeviceScope ds(device_);
      if (!has_snap_) throw std::runtime_error("restore(): no snapshot taken");
+     settle();
+     collect_retired();
      if (n_snap_ > cap_state_) {
-          check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-          check(hipFree(d_state_), "hipFree");
-          d_state_ = nullptr;
+          release_buffer(d_state_);
           cap_state_ = 0;
           check(hipMalloc(&d_state_, n_snap_ * amp_bytes(prec_)), "hipMalloc(state)");
           cap_state_ = n_snap_;
@@ -933,6 +1027,7 @@ void Engine::compare_snapshot(double out[5])
      DeviceScope ds(device_);
      if (!has_snap_ || n_snap_ != n_amps_)
           throw std::runtime_error("compare_snapshot(): no snapshot of the current size");
+     settle();
      begin_timed(kReduce);
      check(launch_compare(d_state_, d_snap_, prec_, n_amps_, d_scratch_,
                           d_scratch_ + kReduceScratchDoubles, stream_),
@@ -945,6 +1040,7 @@ void Engine::compare_snapshot(double out[5])
 void Engine::init_random_state(uint64_t seed)
 {
      DeviceScope ds(device_);
+     settle();
      begin_timed(kFillCopy);
      check(launch_random_state(d_state_, prec_, n_amps_, seed, stream_), "random_state");
      end_timed();
@@ -961,62 +1057,83 @@ int Engine::apply_tiled(const std::vector<Gate>& gates, int op_width)
      DeviceScope ds(device_);
      if (op_width < 1 || op_width > 5) throw std::runtime_error("apply_tiled(): op_width must be 1..5");
      run();
+     settle();
      if (gates.empty()) return 0;
      const int n = static_cast<int>(n_qubits());
      // bit positions of every gate
-     GateList gl;
+     std::vector<PosGate> items;
+     items.reserve(gates.size());
      for (const auto& g: gates) {
           if (g.targets.empty() || g.targets.size() + g.ctrls.size() > 5)
                throw std::runtime_error("apply_tiled(): a gate needs 1..5 qubits with its controls");
           if (g.m.dim != (size_t(1) << g.targets.size()))
                throw std::runtime_error("apply_tiled(): matrix size does not match targets");
-          std::vector<int64_t> t, c;
-          for (auto id: g.targets) t.push_back(static_cast<int64_t>(find_sure(id)));
-          for (auto id: g.ctrls) c.push_back(static_cast<int64_t>(find_sure(id)));
-          std::vector<int64_t> all = t;
-          all.insert(all.end(), c.begin(), c.end());
+          PosGate pg;
+          for (auto id: g.targets) pg.t.push_back(static_cast<int64_t>(find_sure(id)));
+          for (auto id: g.ctrls) pg.c.push_back(static_cast<int64_t>(find_sure(id)));
+          std::vector<int64_t> all = pg.t;
+          all.insert(all.end(), pg.c.begin(), pg.c.end());
           std::sort(all.begin(), all.end());
           if (std::adjacent_find(all.begin(), all.end()) != all.end())
                throw std::runtime_error("apply_tiled(): repeated qubit");
-          gl.gate.push_back(std::move(t));
-          gl.ctrl.push_back(std::move(c));
+          pg.m = g.m;
+          items.push_back(std::move(pg));
+     }
+     if (n >= kTileBits) return tile_items(items, op_width, 0);
+     // small states: every op is one plain fused pass
+     GateList gl;
+     for (const auto& it: items) {
+          gl.gate.push_back(it.t);
+          gl.ctrl.push_back(it.c);
           gl.diag.push_back(false);
      }
-     auto fuse = [&](const std::vector<int>& idx, Mat& M, std::vector<int64_t>& pos) {
+     const std::vector<int> cl = plan_clusters(gl, op_width, kPlanLookahead);
+     const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
+     std::vector<std::vector<int>> members(nc);
+     for (size_t i = 0; i < cl.size(); ++i) members[cl[i]].push_back(static_cast<int>(i));
+     for (const auto& mem: members) {
           Fusion f;
-          for (int i: idx) {
-               Mat m = gates[i].m;
-               std::vector<int64_t> ids = gl.gate[i];
-               add_controls(m, ids, gl.ctrl[i]);
+          for (int i: mem) {
+               Mat m = items[i].m;
+               std::vector<int64_t> ids = items[i].t;
+               add_controls(m, ids, items[i].c);
                f.insert(std::move(m), false, std::move(ids), {});
           }
-          std::vector<int64_t> ctrls;
+          Mat M;
+          std::vector<int64_t> pos, ctrls;
           bool diag = false;
           f.perform(M, pos, ctrls, diag);
-     };
-     if (n < kTileBits) {
-          // small states: every op is one plain fused pass
-          const std::vector<int> cl = plan_clusters(gl, op_width, kPlanLookahead);
-          const int nc = *std::max_element(cl.begin(), cl.end()) + 1;
-          std::vector<std::vector<int>> members(nc);
-          for (size_t i = 0; i < cl.size(); ++i) members[cl[i]].push_back(static_cast<int>(i));
-          for (const auto& mem: members) {
-               Mat M;
-               std::vector<int64_t> pos;
-               fuse(mem, M, pos);
-               std::vector<uint32_t> p32(pos.begin(), pos.end());
-               dense_or_diag(M, p32, 0, false);
-          }
-          return nc;
+          std::vector<uint32_t> p32(pos.begin(), pos.end());
+          dense_or_diag(M, p32, 0, false);
      }
-     const std::vector<TilePlan> plans = plan_tile_passes(gl, n, kTileHigh, op_width);
-     // host image: per pass its ops, then all matrices
+     return nc;
+}
+
+int Engine::tile_items(std::vector<PosGate>& items, int op_width, size_t keep)
+{
+     const int n = n_qubits();
+     if (n < kTileBits) throw std::runtime_error("tile_items(): the state is smaller than a tile");
+     GateList gl;
+     for (const auto& it: items) {
+          gl.gate.push_back(it.t);
+          gl.ctrl.push_back(it.c);
+          gl.diag.push_back(false);
+     }
+     TilePassPlanner planner(gl, n, kTileHigh, op_width);
+     std::vector<TilePlan> plans;
+     {
+          TilePlan tp;
+          while (planner.remaining().size() > keep && planner.next(tp)) plans.push_back(std::move(tp));
+     }
+     if (plans.empty()) return 0;
+     // host image: per pass its ops, then all matrices (fused in FP64)
      const size_t ab = amp_bytes(prec_);
      std::vector<TileOp> ops;
      std::vector<size_t> first;
      std::vector<int> kmax;
      std::vector<char> mats;
      uint64_t entries = 0;
+     std::vector<char> used(items.size(), 0);
      for (const auto& tp: plans) {
           first.push_back(ops.size());
           int km = 1;
@@ -1025,9 +1142,18 @@ int Engine::apply_tiled(const std::vector<Gate>& gates, int op_width)
           for (int p = 0; p < 6; ++p) rank[p] = p;
           for (int j = 0; j < kTileHigh; ++j) rank[tp.hpos[j]] = 6 + j;
           for (const auto& op: tp.ops) {
+               Fusion f;
+               for (int i: op) {
+                    used[i] = 1;
+                    Mat m = items[i].m;
+                    std::vector<int64_t> ids = items[i].t;
+                    add_controls(m, ids, items[i].c);
+                    f.insert(std::move(m), false, std::move(ids), {});
+               }
                Mat M;
-               std::vector<int64_t> pos;
-               fuse(op, M, pos);
+               std::vector<int64_t> pos, ctrls;
+               bool diag = false;
+               f.perform(M, pos, ctrls, diag);
                TileOp to{};
                to.k = static_cast<uint8_t>(pos.size());
                if (to.k < 1 || to.k > 5) throw std::runtime_error("apply_tiled(): op width out of range");
@@ -1058,22 +1184,27 @@ int Engine::apply_tiled(const std::vector<Gate>& gates, int op_width)
      }
      const size_t ops_bytes = (ops.size() * sizeof(TileOp) + 255) / 256 * 256;
      const size_t need = ops_bytes + mats.size();
-     if (prog_busy_) check(hipEventSynchronize(prog_ev_), "hipEventSynchronize");
-     prog_busy_ = false;
-     if (need > prog_cap_) {
-          if (h_prog_) check(hipHostFree(h_prog_), "hipHostFree");
-          if (d_prog_) check(hipFree(d_prog_), "hipFree");
-          h_prog_ = d_prog_ = nullptr;
-          prog_cap_ = std::max(need, 2 * prog_cap_);
-          check(hipHostMalloc(reinterpret_cast<void**>(&h_prog_), prog_cap_, hipHostMallocDefault), "hipHostMalloc(program)");
-          check(hipMalloc(reinterpret_cast<void**>(&d_prog_), prog_cap_), "hipMalloc(program)");
+     // staging slot: waits only for the launches of kProgSlots groups ago
+     ProgSlot& ps = prog_[prog_next_];
+     prog_next_ = (prog_next_ + 1) % kProgSlots;
+     if (ps.busy) check(hipEventSynchronize(ps.ev), "hipEventSynchronize");
+     ps.busy = false;
+     if (need > ps.cap) {
+          if (ps.h) check(hipHostFree(ps.h), "hipHostFree");
+          if (ps.d) check(hipFree(ps.d), "hipFree");
+          ps.h = ps.d = nullptr;
+          ps.cap = 0;
+          const size_t cap = std::max(need, size_t(1) << 20);
+          check(hipHostMalloc(reinterpret_cast<void**>(&ps.h), cap, hipHostMallocDefault), "hipHostMalloc(program)");
+          check(hipMalloc(reinterpret_cast<void**>(&ps.d), cap), "hipMalloc(program)");
+          ps.cap = cap;
      }
-     if (!prog_ev_) check(hipEventCreateWithFlags(&prog_ev_, hipEventDisableTiming), "hipEventCreate");
-     std::memcpy(h_prog_, ops.data(), ops.size() * sizeof(TileOp));
-     std::memcpy(h_prog_ + ops_bytes, mats.data(), mats.size());
-     check(hipMemcpyAsync(d_prog_, h_prog_, need, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
-     const TileOp* d_ops = reinterpret_cast<const TileOp*>(d_prog_);
-     const void* d_mats = d_prog_ + ops_bytes;
+     if (!ps.ev) check(hipEventCreateWithFlags(&ps.ev, hipEventDisableTiming), "hipEventCreate");
+     std::memcpy(ps.h, ops.data(), ops.size() * sizeof(TileOp));
+     std::memcpy(ps.h + ops_bytes, mats.data(), mats.size());
+     check(hipMemcpyAsync(ps.d, ps.h, need, hipMemcpyHostToDevice, stream_), "hipMemcpyAsync(program)");
+     const TileOp* d_ops = reinterpret_cast<const TileOp*>(ps.d);
+     const void* d_mats = ps.d + ops_bytes;
      for (size_t p = 0; p < plans.size(); ++p) {
           TileHigh th{};
           for (int j = 0; j < kTileHigh; ++j) th.hpos[j] = static_cast<uint8_t>(plans[p].hpos[j]);
@@ -1084,8 +1215,14 @@ int Engine::apply_tiled(const std::vector<Gate>& gates, int op_width)
                 "tile pass");
           end_timed();
      }
-     check(hipEventRecord(prog_ev_, stream_), "hipEventRecord");
-     prog_busy_ = true;
+     check(hipEventRecord(ps.ev, stream_), "hipEventRecord");
+     ps.busy = true;
+     // the launched items leave the list; the rest keep program order
+     std::vector<PosGate> rest;
+     rest.reserve(items.size());
+     for (size_t i = 0; i < items.size(); ++i)
+          if (!used[i]) rest.push_back(std::move(items[i]));
+     items.swap(rest);
      return static_cast<int>(plans.size());
 }
 
@@ -1096,6 +1233,7 @@ void Engine::sample(uint32_t shots, uint64_t seed, uint64_t* out)
      if (shots > (1u << 20)) throw std::runtime_error("sample(): at most 2^20 shots");
      if (!out) throw std::runtime_error("sample(): null output");
      run();
+     settle();
      int n = 0;
      while ((uint64_t(1) << n) < n_amps_) ++n;
      if (shots > samples_cap_) {
@@ -1154,6 +1292,7 @@ cplx Engine::expectation(const std::vector<uint64_t>& xm, const std::vector<uint
      DeviceScope ds(device_);
      if (xm.size() != zm.size() || xm.size() != coeff.size())
           throw std::runtime_error("expectation(): term arrays differ in length");
+     settle();
      std::vector<uint32_t> pos(ids.size());
      for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
      auto to_bits = [&](uint64_t m) {
@@ -1212,6 +1351,7 @@ void Engine::apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<
           X[t] = to_bits(xm[t]);
           Z[t] = to_bits(zm[t]);
      }
+     settle();
      ensure_tmp(n_amps_);
      begin_timed(kFillCopy);
      check(launch_fill(d_tmp_, prec_, n_amps_, 0.0, 0.0, stream_), "fill");
@@ -1224,7 +1364,7 @@ void Engine::apply_pauli_sum(const std::vector<uint64_t>& xm, const std::vector<
                 "pauli_accumulate");
           end_timed();
      }
-     swap_tmp();
+     commit_tmp();
 }
 
 void Engine::time_evolution(const std::vector<uint64_t>& xm, const std::vector<uint64_t>& zm,
@@ -1236,6 +1376,7 @@ void Engine::time_evolution(const std::vector<uint64_t>& xm, const std::vector<u
           throw std::runtime_error("emulate_time_evolution(): term arrays differ in length");
      if (!std::isfinite(t)) throw std::runtime_error("emulate_time_evolution(): time must be finite");
      run();
+     settle();
      std::vector<uint32_t> pos(ids.size());
      for (size_t i = 0; i < ids.size(); ++i) pos[i] = static_cast<uint32_t>(find_sure(ids[i]));
      uint64_t cmask = 0, tmask = 0;

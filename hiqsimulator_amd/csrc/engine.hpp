@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <complex>
 #include <cstdint>
 #include <map>
@@ -55,8 +56,20 @@ public:
      void deallocate_qubit(int64_t id);
      // SimulatorMPI.cpp:710-815.
      void apply_gate(Mat m, std::vector<int64_t> ids, const std::vector<int64_t>& ctrls);
-     // SimulatorMPI.cpp:441-541.
+     // SimulatorMPI.cpp:441-541.  Fuses the pending gates into one cluster.
+     // In kPassEach mode the cluster is applied at once as one HBM pass, as
+     // the reference does.  In kPassDeferred mode (the default) a cluster of
+     // at most 5 qubits with its controls, on a state of 12 or more qubits,
+     // is queued instead; queued clusters run as tile passes (several
+     // clusters per HBM pass, planned with the tile as the local set) once
+     // kQueueHigh of them wait, and all of them before any call that reads
+     // the state, changes the qubit set or order, or synchronizes.  The
+     // width check (at most 5 qubits) still raises here.
      void run();
+     enum PassMode : int { kPassEach = 0, kPassDeferred = 1 };
+     void set_pass_mode(int mode);
+     int pass_mode() const { return pass_mode_; }
+     size_t queued_clusters() const { return queue_.size(); }
      // SimulatorMPI.cpp:897-1008.
      std::vector<uint8_t> measure(const std::vector<int64_t>& ids);
      // SimulatorMPI.cpp:561-600.
@@ -83,8 +96,20 @@ public:
      Prec precision() const { return prec_; }
      void get_state_c128(std::complex<double>* out);
      // The state's device buffer after pending gates have run and the
-     // engine's stream has drained; valid until the qubit count changes.
+     // engine's stream has drained.  Valid until the next call that moves the
+     // state to another buffer: allocation, deallocation, permute_qubits,
+     // apply_pauli_sum, restore.
      void* device_state();
+     // device_state() with a pin: while at least one pin is held, results of
+     // permute_qubits and apply_pauli_sum are copied back into the same
+     // buffer (one extra streaming pass each) and no state buffer is freed,
+     // so the pointer stays allocated until the pin is released and follows
+     // the state until the qubit count changes.  unpin_state() only
+     // decrements the pin count (it may be called from any thread); buffers
+     // retired meanwhile are freed by a later call on the engine.
+     void* pin_state();
+     void unpin_state();
+     int view_pins() const { return view_pins_.load(); }
      void set_state_c128(const std::complex<double>* in, uint64_t n);
      // Save/restore of the interleaved state to a file, streamed through a
      // pinned staging buffer (no host copy of the whole state).  Format:
@@ -143,6 +168,7 @@ public:
      void apply_matrix(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask,
                        bool diag);
 
+     // Runs the queued clusters and waits for all device work.
      void synchronize();
      void set_timing(bool on);
      void kernel_time(int cls, uint64_t& count, double& total_ms);
@@ -155,6 +181,26 @@ private:
           hipEvent_t start, stop;
      };
 
+     // A gate on bit positions: targets t (matrix bit l <-> t[l]) and
+     // controls c.
+     struct PosGate {
+          Mat m;
+          std::vector<int64_t> t, c;
+     };
+     // Runs every queued cluster (tile passes).
+     void settle() { drain_queue(0); }
+     // Applies queued clusters as tile passes until at most `keep` remain.
+     void drain_queue(size_t keep);
+     // Plans `items` into tile passes and launches them while more than
+     // `keep` items are left unplanned; the launched items are removed from
+     // `items` (the rest keep program order).  Needs n >= kTileBits.
+     // Returns the number of HBM passes launched.
+     int tile_items(std::vector<PosGate>& items, int op_width, size_t keep);
+     void release_buffer(void*& p);
+     void collect_retired();
+     // A same-size out-of-place result in d_tmp_ becomes the state: the
+     // buffers swap, or (while a pin is held) the result is copied back.
+     void commit_tmp();
      size_t find(int64_t id) const;
      size_t find_sure(int64_t id) const;
      void ensure_state(uint64_t n_amps);
@@ -212,11 +258,26 @@ private:
      size_t stage_bytes_ = 0;
      uint64_t* d_samples_ = nullptr;  // sample outcomes
      size_t samples_cap_ = 0;
-     char* h_prog_ = nullptr;  // pinned staging of tile programs
-     char* d_prog_ = nullptr;
-     size_t prog_cap_ = 0;
-     hipEvent_t prog_ev_ = nullptr;
-     bool prog_busy_ = false;
+     // Tile-program staging ring: pinned host image and device copy of
+     // each launch group's ops and matrices.
+     struct ProgSlot {
+          char* h = nullptr;
+          char* d = nullptr;
+          size_t cap = 0;
+          hipEvent_t ev = nullptr;
+          bool busy = false;
+     };
+     static constexpr int kProgSlots = 3;
+     ProgSlot prog_[kProgSlots];
+     int prog_next_ = 0;
+
+     // Deferred clusters (kPassDeferred): fused matrices on bit positions.
+     static constexpr size_t kQueueHigh = 64, kQueueLow = 32;
+     int pass_mode_ = kPassDeferred;
+     std::vector<PosGate> queue_;
+
+     std::atomic<int> view_pins_{0};
+     std::vector<void*> retired_;
 
      bool timing_ = false;
      std::vector<TimedLaunch> pending_;

@@ -17,6 +17,8 @@
 // into SGPRs, which v_fma_f64 / v_fma_f32 take directly as an operand.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 
 #include "device_common.hpp"
@@ -590,20 +592,22 @@ constexpr int tile_wpb()
 // form.  With the swizzled block order the tile form is as fast or faster
 // at every placement measured on MI355X (30 qubits, k = 1..4), so the
 // default never picks the direct form for that reason.
-int g_high_target = 64;
+// The knobs are process-wide and may be set from any thread while other
+// threads launch: each is an atomic read once per launch.
+std::atomic<int> g_high_target{64};
 // Low block-index bits reversed by swizzle_block: for passes of k <= 3 (and
 // the other streaming kernels) and for passes of k >= 4.  Measured on a
 // 30-qubit k = 4 pass over 30 placements: 5421 GB/s average without
 // swizzle, 5821 with 8 bits, 5940 with 6 bits.
-int g_swizzle_bits = 8;
-int g_swizzle_bits_wide = 6;
-int g_tile_mfma = 1;
-int g_dense5_tile = 1;
+std::atomic<int> g_swizzle_bits{8};
+std::atomic<int> g_swizzle_bits_wide{6};
+std::atomic<int> g_tile_mfma{1};
+std::atomic<int> g_dense5_tile{1};
 // 4-qubit passes with every target at bit 13 or above take the direct form
 // in 1024-thread blocks (16 KiB contiguous per target row): 6.34 against
 // 6.63 ms in the tile form on a 30-qubit state, 32 such placements.
-int g_high_target_wide = 13;
-int g_direct_block = 1024;
+std::atomic<int> g_high_target_wide{13};
+std::atomic<int> g_direct_block{1024};
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -656,7 +660,7 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      // occupancy; the direct form is faster there unless bit 0 (or, for
      // complex64, bit 0 or 1) is a target.
      const bool tile_pays = K < 5 || min_t <= (sizeof(V) == 16 ? 0 : 1);
-     if (tileable && tile_pays && min_t < (K == 4 ? g_high_target_wide : g_high_target)) {
+     if (tileable && tile_pays && min_t < get_tile_high_target_for(K)) {
           TileDesc L{};
           uint64_t lanes = 0;
           int nb = 0;
@@ -695,8 +699,9 @@ hipError_t dense_k(void* psi, int n, const void* mat, const DenseDesc& d, hipStr
      const int line_bits = sizeof(V) == 16 ? 3 : 4;
      const bool nt = (ins_mask & ((1ull << line_bits) - 1)) == 0;
      if constexpr (K == 4) {
-          if (g_direct_block == 512) return dense_direct_launch<K, R, GPT, 512>(psi, mat, d, ngroups, nt, s);
-          if (g_direct_block == 1024) return dense_direct_launch<K, R, GPT, 1024>(psi, mat, d, ngroups, nt, s);
+          const int bs = get_direct_block();
+          if (bs == 512) return dense_direct_launch<K, R, GPT, 512>(psi, mat, d, ngroups, nt, s);
+          if (bs == 1024) return dense_direct_launch<K, R, GPT, 1024>(psi, mat, d, ngroups, nt, s);
      }
      const uint64_t dblocks = (ngroups + kBlock * GPT - 1) / (kBlock * GPT);
      const dim3 grid = grid_2d(dblocks);
@@ -783,21 +788,26 @@ hipError_t reduce_launch(uint64_t work, double* scratch, double* out, hipStream_
 
 }  // namespace
 
-void set_tile_high_target(int pos) { g_high_target = pos; }
-int get_tile_high_target() { return g_high_target; }
-void set_block_swizzle(int bits) { g_swizzle_bits = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
-int get_block_swizzle() { return g_swizzle_bits; }
-void set_block_swizzle_wide(int bits) { g_swizzle_bits_wide = bits < 0 ? 0 : (bits > 16 ? 16 : bits); }
-int get_block_swizzle_wide() { return g_swizzle_bits_wide; }
-int get_block_swizzle_for(int k) { return k >= 4 ? g_swizzle_bits_wide : g_swizzle_bits; }
-void set_tile_mfma(int mode) { g_tile_mfma = mode ? 1 : 0; }
-int get_tile_mfma() { return g_tile_mfma; }
-void set_dense5_tile(int on) { g_dense5_tile = on ? 1 : 0; }
-int get_dense5_tile() { return g_dense5_tile; }
-void set_direct_block(int bs) { g_direct_block = bs; }
-void set_tile_high_target_wide(int pos) { g_high_target_wide = pos; }
-int get_tile_high_target_wide() { return g_high_target_wide; }
-int get_direct_block() { return g_direct_block; }
+namespace {
+constexpr auto kRelaxed = std::memory_order_relaxed;
+}  // namespace
+
+void set_tile_high_target(int pos) { g_high_target.store(pos, kRelaxed); }
+int get_tile_high_target() { return g_high_target.load(kRelaxed); }
+void set_tile_high_target_wide(int pos) { g_high_target_wide.store(pos, kRelaxed); }
+int get_tile_high_target_wide() { return g_high_target_wide.load(kRelaxed); }
+int get_tile_high_target_for(int k) { return k == 4 ? get_tile_high_target_wide() : get_tile_high_target(); }
+void set_block_swizzle(int bits) { g_swizzle_bits.store(bits < 0 ? 0 : (bits > 16 ? 16 : bits), kRelaxed); }
+int get_block_swizzle() { return g_swizzle_bits.load(kRelaxed); }
+void set_block_swizzle_wide(int bits) { g_swizzle_bits_wide.store(bits < 0 ? 0 : (bits > 16 ? 16 : bits), kRelaxed); }
+int get_block_swizzle_wide() { return g_swizzle_bits_wide.load(kRelaxed); }
+int get_block_swizzle_for(int k) { return k >= 4 ? get_block_swizzle_wide() : get_block_swizzle(); }
+void set_tile_mfma(int mode) { g_tile_mfma.store(mode ? 1 : 0, kRelaxed); }
+int get_tile_mfma() { return g_tile_mfma.load(kRelaxed); }
+void set_dense5_tile(int on) { g_dense5_tile.store(on ? 1 : 0, kRelaxed); }
+int get_dense5_tile() { return g_dense5_tile.load(kRelaxed); }
+void set_direct_block(int bs) { g_direct_block.store(bs, kRelaxed); }
+int get_direct_block() { return g_direct_block.load(kRelaxed); }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

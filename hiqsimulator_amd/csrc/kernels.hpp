@@ -39,9 +39,14 @@ hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat,
                         const DenseDesc& d, hipStream_t s);
 
 // The dense pass takes its direct form (instead of the LDS tile form) when
-// every target bit is at or above this position (default 20).
+// every target bit is at or above this position: `tile_high_target` for
+// passes of k != 4 (default 64: never), `tile_high_target_wide` for k = 4
+// (default 13).  get_tile_high_target_for(k) is the one that applies to a
+// k-qubit pass, in single-state, batched and request-loop passes alike.
+// Every knob below is process-wide and atomic (read once per launch).
 void set_tile_high_target(int pos);
 int get_tile_high_target();
+int get_tile_high_target_for(int k);
 // Low block-index bits reversed in the streaming passes (0 = identity
 // order, at most 16): `block_swizzle` for passes of k <= 3 and the other
 // streaming kernels (default 8), `block_swizzle_wide` for dense passes of

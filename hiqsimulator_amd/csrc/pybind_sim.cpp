@@ -55,12 +55,14 @@ constexpr uint8_t kDLComplex = 5;
 struct StateView {
      DLManagedTensor t{};
      int64_t shape[1] = {0};
-     PyObject* owner = nullptr;
+     PyObject* owner = nullptr;  // the SimulatorMPI object: keeps the handle alive
+     hiqsim_t* handle = nullptr;
 };
 
 void state_view_deleter(DLManagedTensor* t)
 {
      auto* v = static_cast<StateView*>(t->manager_ctx);
+     if (v->handle) hiqsim_unpin_device_state(v->handle);  // only a counter
      {
           py::gil_scoped_acquire g;
           Py_XDECREF(v->owner);
@@ -252,6 +254,18 @@ public:
           py::gil_scoped_release r;
           ok(hiqsim_run(h_));
      }
+     void set_pass_mode(int mode)
+     {
+          py::gil_scoped_release r;
+          ok(hiqsim_set_pass_mode(h_, mode));
+     }
+     py::tuple pass_mode()
+     {
+          int mode = 0;
+          size_t queued = 0;
+          ok(hiqsim_get_pass_mode(h_, &mode, &queued));
+          return py::make_tuple(mode, queued);
+     }
      double entropy()
      {
           double e = 0;
@@ -272,8 +286,11 @@ public:
           return a;
      }
      // The state without a copy, as a DLPack capsule ("dltensor") for
-     // torch.from_dlpack: 2^n complex128/complex64 amplitudes on the GPU,
-     // valid until the qubit count changes.
+     // torch.from_dlpack: 2^n complex128/complex64 amplitudes on the GPU.
+     // The view pins the buffer (hiqsim_pin_device_state) until the tensor
+     // is freed: permutations and Pauli sums then write their results back
+     // into it, and the memory stays allocated; the view follows the state
+     // until the qubit count changes.
      static py::capsule state_dlpack(py::object self_obj)
      {
           PySim& self = self_obj.cast<PySim&>();
@@ -282,9 +299,10 @@ public:
           int prec = 0, dev = 0;
           {
                py::gil_scoped_release r;
-               ok(hiqsim_device_state(self.h_, &ptr, &n, &prec, &dev));
+               ok(hiqsim_pin_device_state(self.h_, &ptr, &n, &prec, &dev));
           }
           auto* v = new StateView();
+          v->handle = self.h_;
           v->shape[0] = static_cast<int64_t>(n);
           v->t.dl_tensor.data = ptr;
           v->t.dl_tensor.device = DLDevice{kDLROCM, dev};
@@ -716,6 +734,14 @@ public:
                                   xmask.size(), &id));
           return id;
      }
+     int64_t submit_state(int n, const py::list& gates)
+     {
+          const GateCsr g = gate_csr(gates);
+          int64_t id = -1;
+          ok(hiqsim_server_submit(h_, n, g.size(), g.mats.data(), g.tid.data(), g.toff.data(),
+                                  g.cid.data(), g.coff.data(), 2, 0, 0, nullptr, nullptr, 0, &id));
+          return id;
+     }
      std::vector<int64_t> step(bool block)
      {
           std::vector<int64_t> done(max_active_ + 16);
@@ -753,6 +779,14 @@ public:
           ok(hiqsim_server_expectations(h_, id, nullptr, 0, &n));
           py::array_t<std::complex<double>> a(static_cast<py::ssize_t>(n));
           ok(hiqsim_server_expectations(h_, id, reinterpret_cast<double*>(a.mutable_data()), n, &n));
+          return a;
+     }
+     py::array_t<std::complex<double>> state(int64_t id) const
+     {
+          uint64_t n = 0;
+          ok(hiqsim_server_state(h_, id, nullptr, 0, &n));
+          py::array_t<std::complex<double>> a(static_cast<py::ssize_t>(n));
+          ok(hiqsim_server_state(h_, id, reinterpret_cast<double*>(a.mutable_data()), n, &n));
           return a;
      }
      py::tuple times(int64_t id) const
@@ -807,6 +841,8 @@ PYBIND11_MODULE(_cppsim_amd, m)
          .def("get_amplitude", &PySim::get_amplitude)
          .def("get_probability", &PySim::get_probability)
          .def("run", &PySim::run)
+         .def("set_pass_mode", &PySim::set_pass_mode)
+         .def("pass_mode", &PySim::pass_mode)
          .def("entropy", &PySim::entropy)
          .def("cheat_local", &PySim::cheat_local)
          .def("collapse_wavefunction", &PySim::collapse_wavefunction)
@@ -880,11 +916,13 @@ PYBIND11_MODULE(_cppsim_amd, m)
               py::arg("shots") = 1024, py::arg("seed") = 0)
          .def("submit_expectation", &PyServer::submit_expectation, py::arg("n_qubits"),
               py::arg("gates"), py::arg("xmask"), py::arg("zmask"))
+         .def("submit_state", &PyServer::submit_state, py::arg("n_qubits"), py::arg("gates"))
          .def("step", &PyServer::step, py::arg("block") = false)
          .def("idle", &PyServer::idle)
          .def("done", &PyServer::done)
          .def("samples", &PyServer::samples)
          .def("expectations", &PyServer::expectations)
+         .def("state", &PyServer::state)
          .def("times", &PyServer::times)
          .def("forget", &PyServer::forget)
          .def("stats", &PyServer::stats);

@@ -71,6 +71,8 @@ public:
      // The next tile pass into `out`; false when every gate is planned.
      bool next(TilePlan& out);
      bool done() const { return remaining_.empty(); }
+     // Indices of the gates not planned yet, in program order.
+     const std::vector<size_t>& remaining() const { return remaining_; }
 
 private:
      int score(uint64_t S) const;

@@ -345,6 +345,17 @@ void Server::harvest(bool wait_oldest, std::vector<int64_t>& completed)
                     r.res_samples.resize(r.shots);
                     std::memcpy(r.res_samples.data(), src, r.shots * sizeof(uint64_t));
                }
+               else if (r.kind == ServeKind::State) {
+                    const uint64_t na = uint64_t(1) << r.n;
+                    r.res_state.resize(na);
+                    if (prec_ == Prec::C128) {
+                         std::memcpy(r.res_state.data(), src, na * 16);
+                    }
+                    else {
+                         const float* f = reinterpret_cast<const float*>(src);
+                         for (uint64_t i = 0; i < na; ++i) r.res_state[i] = std::complex<double>(f[2 * i], f[2 * i + 1]);
+                    }
+               }
                else {
                     // <X^x Z^z> from the kernel times i^{|x & z|} (Y = i X Z)
                     static const std::complex<double> ip[4] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
@@ -485,7 +496,7 @@ std::vector<int64_t> Server::step(bool block)
           for (int l = 0; l < pps_ && r.next + l < r.passes.size(); ++l) {
                const Pass& p = r.passes[r.next + l];
                const uint64_t moff = r.mat_off + r.pass_mat[r.next + l];
-               const bool tile_pays = p.pos[0] < get_tile_high_target() &&
+               const bool tile_pays = p.pos[0] < get_tile_high_target_for(p.k) &&
                                       (p.k < 5 || p.pos[0] <= (prec_ == Prec::C128 ? 0 : 1));
                if (r.state_off + (uint64_t(1) << r.n) > state_cap_amps_ ||
                    moff + (uint64_t(1) << (2 * p.k)) > mat_cap_entries_ || p.k < 1 || p.k > r.n)
@@ -561,6 +572,9 @@ std::vector<int64_t> Server::step(bool block)
                jobs.push_back(j);
                sums_needed += uint64_t(1) << lb;
                out_bytes += align_up(r.shots * sizeof(uint64_t), 256);
+          }
+          else if (r.kind == ServeKind::State) {
+               out_bytes += align_up((uint64_t(1) << r.n) * ab, 256);
           }
           else {
                out_bytes += align_up(r.xmask.size() * 2 * sizeof(double), 256);
@@ -680,6 +694,17 @@ std::vector<int64_t> Server::step(bool block)
      if (out_bytes)
           check(hipMemcpyAsync(slot.h_out, slot.d_out, out_bytes, hipMemcpyDeviceToHost, stream_),
                 "hipMemcpyAsync(outputs)");
+     // States go straight from the pool to their place in the pinned output
+     // buffer, after the copy above (same stream).
+     for (const auto& fo: outs) {
+          const Req& r = reqs_.at(fo.first);
+          if (r.kind != ServeKind::State) continue;
+          const uint64_t bytes = (uint64_t(1) << r.n) * ab;
+          if (fo.second + bytes > slot.out_cap) throw std::runtime_error("Server: internal output bounds check failed");
+          check(hipMemcpyAsync(slot.h_out + fo.second, static_cast<const char*>(d_states_) + r.state_off * ab, bytes,
+                               hipMemcpyDeviceToHost, stream_),
+                "hipMemcpyAsync(state)");
+     }
      check(hipEventRecord(slot.ev, stream_), "hipEventRecord");
      slot.busy = true;
      slot.finishing = outs;
@@ -729,6 +754,14 @@ const std::vector<std::complex<double>>& Server::expectations(int64_t id) const
      const Req& r = reqs_.at(id);
      if (r.kind != ServeKind::Expectation) throw std::runtime_error("Server: request has no expectations");
      return r.res_exp;
+}
+
+const std::vector<std::complex<double>>& Server::state(int64_t id) const
+{
+     if (!done(id)) throw std::runtime_error("Server: request " + std::to_string(id) + " is not done");
+     const Req& r = reqs_.at(id);
+     if (r.kind != ServeKind::State) throw std::runtime_error("Server: request has no state output");
+     return r.res_state;
 }
 
 void Server::times(int64_t id, double* t_submit, double* t_admit, double* t_done) const

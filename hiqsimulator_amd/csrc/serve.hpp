@@ -47,7 +47,9 @@
 
 namespace hq {
 
-enum class ServeKind : int { Samples = 0, Expectation = 1 };
+// Output of a request: computational-basis samples, expectation values of
+// Pauli strings, or the final state itself (2^n amplitudes).
+enum class ServeKind : int { Samples = 0, Expectation = 1, State = 2 };
 
 struct ServeStats {
      uint64_t submitted = 0, completed = 0, steps = 0, launches = 0;
@@ -81,6 +83,8 @@ public:
      bool done(int64_t id) const;
      const std::vector<uint64_t>& samples(int64_t id) const;
      const std::vector<std::complex<double>>& expectations(int64_t id) const;
+     // Final state of a State request, widened to complex128.
+     const std::vector<std::complex<double>>& state(int64_t id) const;
      // steady-clock seconds (CLOCK_MONOTONIC) of submit, admission, completion
      void times(int64_t id, double* t_submit, double* t_admit, double* t_done) const;
      void forget(int64_t id);
@@ -130,6 +134,7 @@ private:
           double t_submit = 0, t_admit = 0, t_done = 0;
           std::vector<uint64_t> res_samples;
           std::vector<std::complex<double>> res_exp;
+          std::vector<std::complex<double>> res_state;
      };
      struct Slot {
           char* h_desc = nullptr;

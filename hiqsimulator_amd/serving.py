@@ -42,7 +42,8 @@ def serving_workload(count, rate, seed, n_range=(12, 28), depth_range=(20, 100),
     """Requests with Poisson arrivals at ``rate`` per second.
 
     Returns a list of dicts: arrival (s from the stream start), n, gates,
-    kind ("samples" | "expectation"), shots, seed, terms [(xmask, zmask)].
+    kind ("samples" | "expectation"; the request loop also takes "state",
+    which returns the final state), shots, seed, terms [(xmask, zmask)].
     """
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -65,6 +66,8 @@ def serving_workload(count, rate, seed, n_range=(12, 28), depth_range=(20, 100),
 def submit(server, req):
     if req["kind"] == "samples":
         return server.submit_samples(req["n"], req["gates"], req["shots"], req["seed"])
+    if req["kind"] == "state":
+        return server.submit_state(req["n"], req["gates"])
     xs = [x for x, _ in req["terms"]]
     zs = [z for _, z in req["terms"]]
     return server.submit_expectation(req["n"], req["gates"], xs, zs)
@@ -73,14 +76,18 @@ def submit(server, req):
 def result(server, rid, req):
     if req["kind"] == "samples":
         return server.samples(rid)
+    if req["kind"] == "state":
+        return server.state(rid)
     return server.expectations(rid)
 
 
 def latency_summary(lat, makespan, count):
+    """p50/p99/mean latency (ms), circuits/s and makespan; ``lat_ms`` holds
+    every request's latency (ms, sorted)."""
     lat = np.sort(np.asarray(lat))
     return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "circuits_per_s": count / makespan,
-            "makespan_s": makespan}
+            "makespan_s": makespan, "lat_ms": [float(x) * 1e3 for x in lat]}
 
 
 def run_server(server, reqs, keep_results=False):

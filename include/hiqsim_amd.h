@@ -62,8 +62,25 @@ int hiqsim_deallocate_qubit(hiqsim_t* h, int64_t id);
  * deferred until hiqsim_run; dim must equal 2^nids. */
 int hiqsim_apply_controlled_gate(hiqsim_t* h, const double* m, size_t dim, const int64_t* ids,
                                  size_t nids, const int64_t* ctrls, size_t nctrls);
-/* run (_cppsim_mpi.cpp:79, SimulatorMPI.cpp:441-541): fuse and apply. */
+/* run (_cppsim_mpi.cpp:79, SimulatorMPI.cpp:441-541): fuse the pending gates
+ * into one cluster and apply it.  Raises the reference's "cannot apply N
+ * qubits gate" for a cluster wider than 5 qubits.  How the cluster is
+ * applied follows the handle's pass mode (hiqsim_set_pass_mode). */
 int hiqsim_run(hiqsim_t* h);
+/* Pass mode of hiqsim_run (no reference counterpart):
+ *   HIQSIM_PASS_EACH (0): every cluster is one HBM pass at once, as the
+ *     reference's Run() (SimulatorMPI.cpp:441-541);
+ *   HIQSIM_PASS_DEFERRED (1, the default): on a state of 12 or more qubits a
+ *     cluster of at most 5 qubits with its controls is queued; queued
+ *     clusters run as tile passes, several clusters per HBM pass, planned
+ *     with the LDS tile as the local set (as hiqsim_apply_tiled), once 64
+ *     of them wait and before any call that reads the state, changes the
+ *     qubit set or the qubit order, or synchronizes.  Results equal the
+ *     reference's within floating-point rounding.
+ * get: *mode, and *queued = clusters waiting (queued may be NULL). */
+enum hiqsim_pass_mode { HIQSIM_PASS_EACH = 0, HIQSIM_PASS_DEFERRED = 1 };
+int hiqsim_set_pass_mode(hiqsim_t* h, int mode);
+int hiqsim_get_pass_mode(hiqsim_t* h, int* mode, size_t* queued);
 
 /* measure_qubits (_cppsim_mpi.cpp:74, SimulatorMPI.cpp:897-1008). */
 int hiqsim_measure_qubits(hiqsim_t* h, const int64_t* ids, size_t n, uint8_t* out);
@@ -105,10 +122,21 @@ int hiqsim_get_state(hiqsim_t* h, double* out, uint64_t n_amps);
  * handle's stream and returns the device buffer of n_amps interleaved
  * amplitudes (complex128 or complex64 per *precision, HIQSIM_C128 /
  * HIQSIM_C64) on HIP device *device.  The pointer stays valid until the
- * next allocation or deallocation of a qubit or the handle's destruction;
- * work the caller queues on other streams must be complete before the next
- * call on the handle.  precision and device may be NULL. */
+ * next call that moves the state to another buffer (allocation or
+ * deallocation of a qubit, hiqsim_permute_qubits, hiqsim_apply_pauli_sum,
+ * hiqsim_restore_state) or the handle's destruction; work the caller queues
+ * on other streams must be complete before the next call on the handle.
+ * precision and device may be NULL. */
 int hiqsim_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precision, int* device);
+/* hiqsim_device_state with a pin (no reference counterpart).  While at least
+ * one pin is held, permutations and Pauli sums copy their result back into
+ * the pinned buffer (one extra streaming pass each) and no state buffer is
+ * freed: the pointer stays allocated until hiqsim_unpin_device_state, and
+ * its contents follow the state until the qubit count changes.  Unpinning
+ * only decrements the count and may be called from any thread; buffers
+ * retired while pinned are freed by a later call on the handle. */
+int hiqsim_pin_device_state(hiqsim_t* h, void** ptr, uint64_t* n_amps, int* precision, int* device);
+int hiqsim_unpin_device_state(hiqsim_t* h);
 /* Save/restore of the state to a file (no reference counterpart; the
  * reference can only copy the state out, cheat_local), streamed through a
  * 64 MiB pinned buffer.  Format: "HQSV", u32 version 1, u32 precision
@@ -306,7 +334,8 @@ int hiqsim_batch_stats(hiqsim_batch_t* b, uint64_t* out, size_t len);
  * No counterpart in the reference.  Requests are circuits from |0...0> (as
  * in a batch) with an output: kind 0 = `shots` samples (Philox seed, as
  * hiqsim_sample), kind 1 = expectation values <P_t> of Pauli strings
- * P_t = prod X^x_t Z^z_t with Y on bits set in both masks (bit i = qubit i).
+ * P_t = prod X^x_t Z^z_t with Y on bits set in both masks (bit i = qubit i),
+ * kind 2 = the final state (2^n amplitudes, index bit i = qubit i).
  * A request's state (2^n amplitudes) and fused matrices live in two device
  * pools sized at creation; requests are admitted in arrival order as the
  * pools allow and advance together, passes_per_step fused passes per step. */
@@ -346,6 +375,9 @@ int hiqsim_server_done(hiqsim_server_t* s, int64_t id, int* done);
 int hiqsim_server_samples(hiqsim_server_t* s, int64_t id, uint64_t* out, size_t cap, size_t* n);
 int hiqsim_server_expectations(hiqsim_server_t* s, int64_t id, double* out, size_t cap_terms,
                                size_t* n);
+/* Final state of a completed kind-2 request as interleaved complex128
+ * (complex64 states are widened); *n_amps = 2^n. */
+int hiqsim_server_state(hiqsim_server_t* s, int64_t id, double* out, uint64_t cap_amps, uint64_t* n_amps);
 /* out[0..2] = steady-clock seconds of submission, admission, completion. */
 int hiqsim_server_times(hiqsim_server_t* s, int64_t id, double* out);
 /* Drops a completed request's results. */

@@ -10,7 +10,9 @@
 // low, an enclosing `#pragma omp parallel`, diag vs dense selection.
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "simulator-mpi/alignedallocator.hpp"
@@ -39,7 +41,48 @@ using nointrin::kernelK_diag1;
 
 namespace {
 using Complex = std::complex<double>;
-using StateVector = std::vector<Complex, aligned_allocator<Complex, 64>>;
+
+// Allocator of the harness's state vector: 64-byte aligned like the
+// reference's StateVector (SimulatorMPI.hpp), but resize() leaves the
+// elements untouched, so that ref_state_fill's OpenMP threads touch the
+// pages first and each thread's static share of the state lands in its own
+// NUMA node, which is how a multi-socket host runs a memory-bound pass at
+// full bandwidth.  (The reference zero-fills from one thread.)
+template <class T>
+struct FirstTouchAlloc {
+     using value_type = T;
+     FirstTouchAlloc() noexcept {}
+     template <class U>
+     FirstTouchAlloc(const FirstTouchAlloc<U>&) noexcept
+     {}
+     T* allocate(size_t n)
+     {
+          void* p = nullptr;
+          if (posix_memalign(&p, 64, n * sizeof(T))) throw std::bad_alloc();
+          return static_cast<T*>(p);
+     }
+     void deallocate(T* p, size_t) noexcept { std::free(p); }
+     template <class U>
+     void construct(U*) noexcept
+     {}
+     template <class U, class A0, class... A>
+     void construct(U* p, A0&& a0, A&&... a)
+     {
+          ::new (static_cast<void*>(p)) U(std::forward<A0>(a0), std::forward<A>(a)...);
+     }
+     template <class U>
+     bool operator==(const FirstTouchAlloc<U>&) const noexcept
+     {
+          return true;
+     }
+     template <class U>
+     bool operator!=(const FirstTouchAlloc<U>&) const noexcept
+     {
+          return false;
+     }
+};
+
+using StateVector = std::vector<Complex, FirstTouchAlloc<Complex>>;
 using Matrix = std::vector<std::vector<Complex, aligned_allocator<Complex, 64>>>;
 
 struct RefState {
@@ -58,11 +101,25 @@ Matrix to_matrix(const double* m, size_t dim)
 
 extern "C" {
 
+// A state of n_amps amplitudes, zero, first touched by the OpenMP threads.
 void* ref_state_new(uint64_t n_amps)
 {
      auto* s = new RefState();
      s->vec.resize(n_amps);
+     Complex* v = s->vec.data();
+     const int64_t n = static_cast<int64_t>(n_amps);
+#pragma omp parallel for schedule(static)
+     for (int64_t i = 0; i < n; ++i) v[i] = Complex(0.0, 0.0);
      return s;
+}
+
+// Every amplitude = (re, im), in parallel with the same static schedule.
+void ref_state_fill(void* s, double re, double im)
+{
+     StateVector& v = static_cast<RefState*>(s)->vec;
+     const int64_t n = static_cast<int64_t>(v.size());
+#pragma omp parallel for schedule(static)
+     for (int64_t i = 0; i < n; ++i) v[i] = Complex(re, im);
 }
 void ref_state_free(void* s) { delete static_cast<RefState*>(s); }
 double* ref_state_data(void* s) { return reinterpret_cast<double*>(static_cast<RefState*>(s)->vec.data()); }

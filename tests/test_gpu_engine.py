@@ -40,12 +40,15 @@ def test_ghz_measurement_matches_oracle(engine_mod, oracle_mod):
     assert seen == {0, 5}
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["pass-each", "deferred"])
 @pytest.mark.parametrize("k", [2, 3, 4, 5])
-def test_config0_random_circuit(engine_mod, oracle_mod, k):
-    """16 qubits, 20 layers, |0..0>, one seed: full state vs oracle."""
+def test_config0_random_circuit(engine_mod, oracle_mod, k, mode):
+    """16 qubits, 20 layers, |0..0>, one seed: full state vs oracle, with
+    one fused pass per run() and with deferred clusters (tile passes)."""
     n, layers, seed = 16, 20, 2024
     gates = random_layers_circuit(n, layers, seed)
     g = make_engine(seed, n, k)
+    g.set_pass_mode(mode)
     o = oracle_mod.OracleSim(seed, n, k)
     for s in (g, o):
         s.allocate_qureg(list(range(n)), 0)
@@ -144,6 +147,54 @@ def test_expectation_and_operator_vs_oracle(engine_mod, oracle_mod):
     assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_entropy_closed_forms(engine_mod, prec):
+    """Entropy (SimulatorMPI.cpp:681-694) against closed forms: a basis state
+    0; H on n qubits n; GHZ 1; a product of Ry(theta_q)|0> the sum of the
+    binary entropies H2(sin^2(theta_q / 2)); probabilities (1/2, 1/4, 1/8,
+    1/8) 1.75.  complex64 amplitudes carry about 1e-7 relative error."""
+    tol = 1e-12 if prec == "c128" else 2e-6
+
+    def fresh(n):
+        s = engine_mod.SimulatorMPI(1, n, 4, prec)
+        s.allocate_qureg(list(range(n)), 0)
+        return s
+
+    def ry(t):
+        c, s = math.cos(t / 2), math.sin(t / 2)
+        return np.array([[c, -s], [s, c]], dtype=complex)
+
+    def h2(p):
+        return sum(-x * math.log2(x) for x in (p, 1.0 - p) if x > 0)
+
+    g = fresh(14)
+    g.apply_controlled_gate(X, [9], [])
+    g.run()
+    assert abs(g.entropy()) < tol
+    for n in (1, 5, 20):
+        g = fresh(n)
+        for q in range(n):
+            g.apply_controlled_gate(H, [q], [])
+            g.run()
+        assert abs(g.entropy() - n) < tol * max(1, n)
+    g = fresh(16)
+    g.apply_controlled_gate(H, [0], [])
+    g.run()
+    for q in range(1, 16):
+        g.apply_controlled_gate(X, [q], [q - 1])
+        g.run()
+    assert abs(g.entropy() - 1.0) < tol
+    thetas = np.random.default_rng(17).uniform(0.1, math.pi - 0.1, size=13)
+    g = fresh(13)
+    for q, t in enumerate(thetas):
+        g.apply_controlled_gate(ry(t), [q], [])
+        g.run()
+    assert abs(g.entropy() - sum(h2(math.sin(t / 2) ** 2) for t in thetas)) < tol * 13
+    g = fresh(2)
+    g.set_state(np.sqrt(np.array([0.5, 0.25, 0.125, 0.125])).astype(complex))
+    assert abs(g.entropy() - 1.75) < tol
+
+
 def test_run_rejects_wide_pending_fusion(engine_mod):
     """Gates left pending across 13 qubits: run() raises the reference's
     "cannot apply" error before building the 2^13 x 2^13 fused matrix
@@ -184,10 +235,12 @@ def test_permutations_keep_results(engine_mod, oracle_mod):
         assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
 
 
-def test_complex64_fidelity(engine_mod, oracle_mod):
+@pytest.mark.parametrize("mode", [0, 1], ids=["pass-each", "deferred"])
+def test_complex64_fidelity(engine_mod, oracle_mod, mode):
     n = 16
     gates = random_layers_circuit(n, 20, seed=5)
     g = engine_mod.SimulatorMPI(5, n, 4, "c64")
+    g.set_pass_mode(mode)
     o = oracle_mod.OracleSim(5, n, 4)
     for s in (g, o):
         s.allocate_qureg(list(range(n)), 0)
@@ -277,6 +330,7 @@ def test_measurement_and_sampling_30q(engine_mod):
     for bits in range(8):
         vals = [(bits >> i) & 1 for i in range(3)]
         assert abs(g.get_probability(vals, [0, 15, 29]) - 0.125) < 1e-12
+    assert abs(g.entropy() - 3.0) < 1e-12  # uniform over 8 outcomes
     s = np.asarray(g.sample(8192, 2024), dtype=np.uint64)
     allowed = {(b & 1) | ((b >> 1 & 1) << 15) | ((b >> 2 & 1) << 29) for b in range(8)}
     assert set(int(x) for x in np.unique(s)) <= allowed
@@ -338,6 +392,7 @@ def test_ghz_expectations_30q(engine_mod):
     assert abs(ev([(q, "X") for q in range(n)]) - 1.0) < 1e-12
     assert abs(ev([(0, "Y"), (1, "Y")] + [(q, "X") for q in range(2, n)]) + 1.0) < 1e-12
     assert abs(ev([(0, "Z")])) < 1e-12
+    assert abs(g.entropy() - 1.0) < 1e-12  # two outcomes of probability 1/2
     g.apply_qubit_operator([([(q, "X") for q in range(n)], 1.0)], ids)
     assert abs(g.get_probability([1] * n, ids) - 0.5) < 1e-12
     assert abs(g.get_amplitude([0] * n, ids) - 1.0 / math.sqrt(2.0)) < 1e-12
@@ -378,3 +433,52 @@ def test_snapshot_restore(engine_mod):
     g.restore()
     assert g.get_qubits_ids() == list(range(n))
     assert np.array_equal(g.get_state(), before)
+
+
+def test_deferred_clusters_flush_at_every_read(engine_mod, oracle_mod):
+    """Clusters queued by run() (deferred mode, 14 qubits) are applied before
+    every call that reads the state or changes the qubit set or order:
+    probabilities, amplitudes, measurement, expectation, permutation,
+    allocation and deallocation all see the same state as the oracle, which
+    applies each cluster at once; a cluster with a 5-qubit control set takes
+    the immediate control-mask pass after the queue."""
+    n = 14
+    gates = random_layers_circuit(n, 6, seed=31)
+    g = make_engine(77, n + 1, 4)
+    o = oracle_mod.OracleSim(77, n + 1, 4)
+    assert g.pass_mode() == (1, 0)
+    for s in (g, o):
+        s.allocate_qureg(list(range(n)), 0)
+        apply_circuit_inorder(s, gates[:60], 4)
+    assert g.pass_mode()[1] > 0  # clusters are waiting
+    ids = list(range(n))
+    assert g.get_probability([1, 0, 1], [0, 5, 13]) == pytest.approx(
+        o.get_probability([1, 0, 1], [0, 5, 13]), abs=1e-13)
+    assert g.pass_mode()[1] == 0
+    for s in (g, o):
+        apply_circuit_inorder(s, gates[60:120], 4)
+        s.apply_controlled_gate(X, [3], [0, 1, 2, 4, 5])  # 6 qubits: control-mask pass
+        s.run()
+    assert g.get_amplitude([0] * n, ids) == pytest.approx(o.get_amplitude([0] * n, ids), abs=1e-13)
+    for s in (g, o):
+        apply_circuit_inorder(s, gates[120:200], 4)
+    assert g.measure_qubits([2, 7]) == o.measure_qubits([2, 7])
+    for s in (g, o):
+        apply_circuit_inorder(s, gates[200:260], 4)
+    term = [([(1, "X"), (4, "Y"), (9, "Z")], 0.7)]
+    assert g.get_expectation_value(term, ids) == pytest.approx(o.get_expectation_value(term, ids), abs=1e-12)
+    perm = random_permutation(ids, np.random.default_rng(3))
+    for s in (g, o):
+        apply_circuit_inorder(s, gates[260:320], 4)
+        s.permute_qubits(perm)
+        apply_circuit_inorder(s, gates[320:], 4)
+        s.allocate_qubit(n)
+        s.apply_controlled_gate(H, [n], [])
+        s.run()
+    assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+    (bit,) = g.measure_qubits([n])
+    assert o.measure_qubits([n]) == [bit]
+    for s in (g, o):
+        s.deallocate_qubit(n)
+    assert np.abs(g.get_state() - o.get_state()).max() <= 1e-12
+    assert g.entropy() == pytest.approx(o.entropy(), abs=1e-12)

@@ -43,6 +43,28 @@ del view
 sim.apply_controlled_gate([[0, 1], [1, 0]], [0], [])
 sim.run()
 assert abs(sim.get_probability([], []) - 1.0) < 1e-6
+# While a view lives the buffer is pinned: a permutation and a Pauli sum
+# (both computed out of place) write their results back into it, so the
+# view keeps following the state.
+view = hq.state_view(sim)
+ptr = view.data_ptr()
+order = list(reversed(sim.get_qubits_ids()))
+sim.permute_qubits(order)
+sim.synchronize()
+assert sim.get_qubits_ids() == order
+assert np.array_equal(view.cpu().numpy().astype(np.complex128), sim.get_state())
+sim.apply_qubit_operator([([(0, "X"), (3, "Z")], 0.5), ([(5, "Y")], 0.25j)], list(range(11)))
+sim.synchronize()
+assert np.array_equal(view.cpu().numpy().astype(np.complex128), sim.get_state())
+assert hq.state_view(sim).data_ptr() == ptr  # a second view shares the pinned buffer
+# a new qubit moves the state to a larger buffer; the old one stays allocated
+# until the view is gone
+sim.allocate_qubit(11)
+sim.synchronize()
+_ = float(view.abs().sum().item())
+del view
+sim.permute_qubits(list(range(12)))
+assert abs(sim.get_probability([], []) - 1.0) < 1e-5
 print("VIEW OK")
 """
 
@@ -51,7 +73,8 @@ print("VIEW OK")
 def test_dlpack_view_aliases_the_state(engine_mod, precision):
     """In a child process that imports torch first (so both libraries share
     torch's HIP runtime): the view equals the copied-out state and aliases
-    it."""
+    it, and it keeps following the state across a permutation and a Pauli
+    sum (its buffer is pinned while it lives)."""
     import os
     import subprocess
     import sys

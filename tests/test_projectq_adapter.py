@@ -138,3 +138,45 @@ def test_is_available_rules():
     assert not sim.is_available(Command(None, TimeEvolution(), ([],)))
     assert sim.is_available(Command(None, MatrixGate(np.eye(32)), ([],)))
     assert not sim.is_available(Command(None, MatrixGate(np.eye(64)), ([],)))
+
+
+def test_resource_counter_counts_qureg_allocation():
+    """A ResourceCounter in a HiQ engine list counts an AllocateQuregGate as
+    one allocation per qubit (the reference's ResourceCounter hook,
+    hiq/projectq/ops/_gates.py:43-73): active qubits, widest point and the
+    gate count; later gates on those qubits find their depth entries."""
+    from projectq.backends import ResourceCounter
+    from projectq.ops import H, Measure
+
+    from hiqsimulator_amd.projectq import AllocateQuregGate, GreedyScheduler, HiQMainEngine
+    counter = ResourceCounter()
+    eng = HiQMainEngine(make_backend("oracle"), [counter, GreedyScheduler()])
+    qureg = eng.allocate_qureg(5)
+    assert counter._active_qubits == 5 and counter.max_width == 5
+    assert sorted(counter._depth_of_qubit) == sorted(q.id for q in qureg)
+    assert counter.gate_counts[(AllocateQuregGate(0), 0)] == 1
+    assert counter.gate_class_counts[(AllocateQuregGate, 0)] == 1
+    qubit = eng.allocate_qubit()
+    assert counter._active_qubits == 6 and counter.max_width == 6
+    H | qureg[2]
+    assert counter._depth_of_qubit[qureg[2].id] == 1
+    Measure | qubit
+    eng.flush()
+    eng.deallocate_qubit(qubit[0])
+    assert counter._active_qubits == 5 and counter.max_width == 6
+
+
+def test_resource_counter_without_hook_miscounts():
+    """The counter's own _add_cmd (the hook bypassed) cannot count a register
+    allocation: it finds no depth entry for the register's qubits."""
+    from projectq.backends import ResourceCounter
+
+    from hiqsimulator_amd.projectq import AllocateQuregGate
+    from projectq.ops import Command
+    from projectq.types import Qubit
+    counter = ResourceCounter()
+    cmd = Command(None, AllocateQuregGate(0), ([Qubit(None, 0), Qubit(None, 1)],))
+    with pytest.raises(KeyError):
+        ResourceCounter._hiq_own_add_cmd(counter, cmd)
+    counter._add_cmd(cmd)
+    assert counter._active_qubits == 2
