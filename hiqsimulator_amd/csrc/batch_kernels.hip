@@ -1149,47 +1149,6 @@ hipError_t launch_serve_layer(void* arena, Prec prec, const void* mats, const Ba
      return hipGetLastError();
 }
 
-bool make_tile_item(int n, int k, const uint8_t* pos, Prec prec, BatchTileItem& it)
-{
-     const int wpb = batch_tile_wpb(k, prec);
-     if (k < 1 || k > 5 || n - k < 6 + (wpb == 4 ? 2 : 1)) return false;
-     uint64_t tmask = 0;
-     for (int l = 0; l < k; ++l) tmask |= 1ull << pos[l];
-     // lanes: the six lowest non-target positions
-     uint64_t lanes = 0;
-     int nb = 0;
-     for (int p = 0; p < n && nb < 6; ++p)
-          if (!((tmask >> p) & 1)) {
-               lanes |= 1ull << p;
-               ++nb;
-          }
-     const uint64_t vbits = lanes | tmask;
-     if ((vbits & 63) != 63) return false;
-     int vpos[64], rank_of[64], nv = 0;
-     for (int p = 0; p < 64; ++p)
-          if ((vbits >> p) & 1) {
-               rank_of[p] = nv;
-               vpos[nv++] = p;
-          }
-     if (nv != 6 + k) return false;
-     nb = 0;
-     for (int p = 0; p < 64; ++p)
-          if ((lanes >> p) & 1) it.rank_n[nb++] = static_cast<uint8_t>(rank_of[p]);
-     for (int l = 0; l < k; ++l) {
-          it.rank_t[l] = static_cast<uint8_t>(rank_of[pos[l]]);
-          it.ins[l] = pos[l];
-     }
-     it.nins = static_cast<uint8_t>(k);
-     it.log_blocks = static_cast<uint8_t>(n - k - 6 - (wpb == 4 ? 2 : 1));
-     for (int i = 0; i < 32; ++i) {
-          uint64_t o = 0;
-          if (i < (1 << k))
-               for (int q = 0; q < k; ++q)
-                    if ((i >> q) & 1) o |= 1ull << vpos[6 + q];
-          it.hoff[i] = static_cast<uint32_t>(o);
-     }
-     return true;
-}
 
 hipError_t launch_batch_tile(void* arena, Prec prec, const void* mats, const BatchTileItem* items,
                              int nitems, int k, uint64_t total_blocks, hipStream_t s)

@@ -20,6 +20,7 @@
 // still be taken, which leaves every result unchanged.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
