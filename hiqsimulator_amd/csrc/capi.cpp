@@ -369,7 +369,7 @@ int hiqsim_get_pass_mode(hiqsim_t* h, int* mode, size_t* queued)
           need(h, "handle");
           need(mode, "mode");
           *mode = h->eng.pass_mode();
-          if (queued) *queued = h->eng.queued_clusters();
+          if (queued) *queued = h->eng.queued_gates();
      });
 }
 
@@ -646,6 +646,14 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                     throw std::runtime_error("direct_block must be 256, 512 or 1024");
                hq::set_direct_block(static_cast<int>(value));
           }
+          else if (n == "permute_swizzle") {
+               if (value < 0 || value > 16) throw std::runtime_error("permute_swizzle must be 0..16");
+               hq::set_permute_swizzle(static_cast<int>(value));
+          }
+          else if (n == "permute_block") {
+               if (value != 256 && value != 512) throw std::runtime_error("permute_block must be 256 or 512");
+               hq::set_permute_block(static_cast<int>(value));
+          }
           else
                throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
@@ -664,6 +672,8 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           else if (n == "dense5_tile") *value = hq::get_dense5_tile();
           else if (n == "direct_block") *value = hq::get_direct_block();
           else if (n == "tile_high_target_wide") *value = hq::get_tile_high_target_wide();
+          else if (n == "permute_swizzle") *value = hq::get_permute_swizzle();
+          else if (n == "permute_block") *value = hq::get_permute_block();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

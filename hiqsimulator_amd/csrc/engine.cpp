@@ -454,6 +454,33 @@ void Engine::run()
      if (fusion_.num_qubits() > 5)
           throw std::runtime_error("Run(): cannot apply " + std::to_string(fusion_.num_qubits())
                                    + " qubits gate");
+     if (pass_mode_ == kPassDeferred && fusion_.num_items() > 0 && n_qubits() >= kTileBits) {
+          // The cluster's gates are queued unfused, each with the shared
+          // controls: the tile planner then packs gates (not whole clusters)
+          // into the tiles, as apply_tiled does.
+          uint64_t cmask = 0;
+          for (auto c: fusion_.shared_controls()) {
+               const size_t p = find(c);
+               if (p != static_cast<size_t>(-1)) cmask |= uint64_t(1) << p;
+          }
+          if (fusion_.num_qubits() + static_cast<size_t>(__builtin_popcountll(cmask)) <= 5) {
+               std::vector<int64_t> cpos;
+               for (int p = 0; p < 64; ++p)
+                    if ((cmask >> p) & 1) cpos.push_back(p);
+               std::vector<Mat> ms;
+               std::vector<std::vector<int64_t>> ids;
+               fusion_.take_items(ms, ids);
+               for (size_t i = 0; i < ms.size(); ++i) {
+                    PosGate g;
+                    g.m = std::move(ms[i]);
+                    for (auto id: ids[i]) g.t.push_back(static_cast<int64_t>(find_sure(id)));
+                    g.c = cpos;
+                    queue_.push_back(std::move(g));
+               }
+               if (queue_.size() >= kQueueHigh) drain_queue(kQueueLow);
+               return;
+          }
+     }
      Mat m;
      std::vector<int64_t> ids, ctrls;
      bool diag = false;
@@ -470,18 +497,6 @@ void Engine::run()
                                    + " qubits gate");
      if (ids.empty() && m(0, 0) == cplx(1.0, 0.0)) {
           fusion_ = Fusion();
-          return;
-     }
-     const size_t nctrl = static_cast<size_t>(__builtin_popcountll(cmask));
-     if (pass_mode_ == kPassDeferred && !ids.empty() && n_qubits() >= kTileBits && ids.size() + nctrl <= 5) {
-          PosGate g;
-          g.m = std::move(m);
-          g.t.assign(pos.begin(), pos.end());
-          for (int p = 0; p < 64; ++p)
-               if ((cmask >> p) & 1) g.c.push_back(p);
-          queue_.push_back(std::move(g));
-          fusion_ = Fusion();
-          if (queue_.size() >= kQueueHigh) drain_queue(kQueueLow);
           return;
      }
      settle();

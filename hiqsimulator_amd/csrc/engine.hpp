@@ -58,18 +58,19 @@ public:
      void apply_gate(Mat m, std::vector<int64_t> ids, const std::vector<int64_t>& ctrls);
      // SimulatorMPI.cpp:441-541.  Fuses the pending gates into one cluster.
      // In kPassEach mode the cluster is applied at once as one HBM pass, as
-     // the reference does.  In kPassDeferred mode (the default) a cluster of
-     // at most 5 qubits with its controls, on a state of 12 or more qubits,
-     // is queued instead; queued clusters run as tile passes (several
-     // clusters per HBM pass, planned with the tile as the local set) once
-     // kQueueHigh of them wait, and all of them before any call that reads
-     // the state, changes the qubit set or order, or synchronizes.  The
-     // width check (at most 5 qubits) still raises here.
+     // the reference does.  In kPassDeferred mode (the default) the gates of
+     // a cluster of at most 5 qubits with its controls, on a state of 12 or
+     // more qubits, are queued instead (unfused, on bit positions); queued
+     // gates run as tile passes (several clusters per HBM pass, planned with
+     // the tile as the local set and fused per tile op) once kQueueHigh of
+     // them wait, and all of them before any call that reads the state,
+     // changes the qubit set or order, or synchronizes.  The width check (at
+     // most 5 qubits) still raises here.
      void run();
      enum PassMode : int { kPassEach = 0, kPassDeferred = 1 };
      void set_pass_mode(int mode);
      int pass_mode() const { return pass_mode_; }
-     size_t queued_clusters() const { return queue_.size(); }
+     size_t queued_gates() const { return queue_.size(); }
      // SimulatorMPI.cpp:897-1008.
      std::vector<uint8_t> measure(const std::vector<int64_t>& ids);
      // SimulatorMPI.cpp:561-600.
@@ -271,8 +272,11 @@ private:
      ProgSlot prog_[kProgSlots];
      int prog_next_ = 0;
 
-     // Deferred clusters (kPassDeferred): fused matrices on bit positions.
-     static constexpr size_t kQueueHigh = 64, kQueueLow = 32;
+     // Deferred gates (kPassDeferred): matrices on bit positions.  Planning
+     // over 256+ gates at a time gives nearly the passes of planning the
+     // whole list (56 against 56 for the 30-qubit, 1780-gate circuit through
+     // the GreedyScheduler loop; 101 when whole clusters are queued).
+     static constexpr size_t kQueueHigh = 512, kQueueLow = 256;
      int pass_mode_ = kPassDeferred;
      std::vector<PosGate> queue_;
 

@@ -119,6 +119,27 @@ public:
           factor_ = 1.0;
      }
 
+     // Controls every item shares (kept as a control mask by perform()).
+     std::vector<int64_t> shared_controls() const { return {ctrl_set_.begin(), ctrl_set_.end()}; }
+
+     // Hands the pending items over unfused, in insertion order: each item's
+     // matrix on its ids (controls not shared by every item already folded
+     // in as diag(I, U)); the global factor is multiplied into the first
+     // item.  The shared controls are those of shared_controls().  Leaves
+     // the fusion empty.  Needs at least one item.
+     void take_items(std::vector<Mat>& ms, std::vector<std::vector<int64_t>>& ids)
+     {
+          ms.clear();
+          ids.clear();
+          for (auto& it: items_) {
+               ms.push_back(std::move(it.m));
+               ids.push_back(std::move(it.ids));
+          }
+          if (!ms.empty() && factor_ != cplx(1.0, 0.0))
+               for (auto& x: ms.front().a) x *= factor_;
+          *this = Fusion();
+     }
+
 private:
      struct Item {
           Mat m;

@@ -278,15 +278,19 @@ __device__ __forceinline__ int perm_slot(int q)
      return q ^ ((q >> 4) & 15) ^ ((q >> 8) & 15);
 }
 
-template <typename R, int T>
-__global__ void __launch_bounds__(kBlock)
+// Workgroups of 2^TB threads; tile `tix` = the block index with its low
+// `swz` bits reversed (swizzle_block), so the blocks in flight spread over
+// far-apart source addresses.
+template <typename R, int T, int TB>
+__global__ void __launch_bounds__(1 << TB)
     k_permute_tile(typename C2<R>::T* __restrict__ dst, const typename C2<R>::T* __restrict__ src,
-                   const PermTile P)
+                   const PermTile P, const int swz)
 {
      using V = typename C2<R>::T;
-     constexpr int E = 1 << (T - 8);
+     constexpr int NT = 1 << TB;
+     constexpr int E = 1 << (T - TB);
      __shared__ V tile[1 << T];
-     const uint64_t dbase = insert_zeros(block_linear(), P.u);
+     const uint64_t dbase = insert_zeros(swizzle_block(block_linear(), swz), P.u);
      uint64_t sbase = 0;
      for (int p = 0; p < P.nq; ++p)
           if ((dbase >> p) & 1) sbase |= 1ull << P.src_pos[p];
@@ -294,7 +298,7 @@ __global__ void __launch_bounds__(kBlock)
      uint64_t slo = 0, dlo = 0;
      int qlo = 0;
 #pragma unroll
-     for (int b = 0; b < 8; ++b) {
+     for (int b = 0; b < TB; ++b) {
           const uint64_t bit = (tid >> b) & 1;
           slo |= bit << P.src_sorted[b];
           dlo |= bit << P.u.p[b];
@@ -305,10 +309,10 @@ __global__ void __launch_bounds__(kBlock)
           uint64_t shi = 0;
           int qhi = 0;
 #pragma unroll
-          for (int b = 0; b < T - 8; ++b) {
+          for (int b = 0; b < T - TB; ++b) {
                const uint64_t bit = (e >> b) & 1;
-               shi |= bit << P.src_sorted[8 + b];
-               qhi |= static_cast<int>(bit) << P.qbit_of_rbit[8 + b];
+               shi |= bit << P.src_sorted[TB + b];
+               qhi |= static_cast<int>(bit) << P.qbit_of_rbit[TB + b];
           }
           tile[perm_slot<V>(qlo | qhi)] = ld<true>(src + sbase + slo + shi);
      }
@@ -317,8 +321,8 @@ __global__ void __launch_bounds__(kBlock)
      for (int e = 0; e < E; ++e) {
           uint64_t dhi = 0;
 #pragma unroll
-          for (int b = 0; b < T - 8; ++b) dhi |= static_cast<uint64_t>((e >> b) & 1) << P.u.p[8 + b];
-          st<true>(dst + dbase + dlo + dhi, tile[perm_slot<V>(e * kBlock + tid)]);
+          for (int b = 0; b < T - TB; ++b) dhi |= static_cast<uint64_t>((e >> b) & 1) << P.u.p[TB + b];
+          st<true>(dst + dbase + dlo + dhi, tile[perm_slot<V>(e * NT + tid)]);
      }
 }
 
@@ -608,6 +612,10 @@ std::atomic<int> g_dense5_tile{1};
 // 6.63 ms in the tile form on a 30-qubit state, 32 such placements.
 std::atomic<int> g_high_target_wide{13};
 std::atomic<int> g_direct_block{1024};
+// Tiled qubit permutation: low block-index bits reversed, and threads per
+// workgroup (256 or 512).
+std::atomic<int> g_permute_swizzle{0};
+std::atomic<int> g_permute_block{256};
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -808,6 +816,10 @@ void set_dense5_tile(int on) { g_dense5_tile.store(on ? 1 : 0, kRelaxed); }
 int get_dense5_tile() { return g_dense5_tile.load(kRelaxed); }
 void set_direct_block(int bs) { g_direct_block.store(bs, kRelaxed); }
 int get_direct_block() { return g_direct_block.load(kRelaxed); }
+void set_permute_swizzle(int bits) { g_permute_swizzle.store(bits < 0 ? 0 : (bits > 16 ? 16 : bits), kRelaxed); }
+int get_permute_swizzle() { return g_permute_swizzle.load(kRelaxed); }
+void set_permute_block(int bs) { g_permute_block.store(bs == 512 ? 512 : 256, kRelaxed); }
+int get_permute_block() { return g_permute_block.load(kRelaxed); }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)
@@ -931,10 +943,20 @@ hipError_t launch_permute(void* dst, const void* src, Prec prec, int n_qubits,
                     P.qbit_of_rbit[a] = static_cast<uint8_t>(rank_u[d_of[b]]);
                     ++a;
                }
-          const dim3 grid = grid_2d(n >> T);
-#define HQ_PERM_TILE(RT, VT_, TT)                                                                 \
-     hipLaunchKernelGGL((k_permute_tile<RT, TT>), grid, dim3(kBlock), 0, s, static_cast<VT_*>(dst), \
-                        static_cast<const VT_*>(src), P)
+          const uint64_t nblocks = n >> T;
+          const dim3 grid = grid_2d(nblocks);
+          int swz = g_permute_swizzle.load(std::memory_order_relaxed);
+          while (swz > 0 && (nblocks % (1ull << swz)) != 0) --swz;
+          const bool wide = g_permute_block.load(std::memory_order_relaxed) == 512;
+#define HQ_PERM_TILE(RT, VT_, TT)                                                                  \
+     do {                                                                                         \
+          if (wide)                                                                               \
+               hipLaunchKernelGGL((k_permute_tile<RT, TT, 9>), grid, dim3(512), 0, s,             \
+                                  static_cast<VT_*>(dst), static_cast<const VT_*>(src), P, swz);   \
+          else                                                                                    \
+               hipLaunchKernelGGL((k_permute_tile<RT, TT, 8>), grid, dim3(256), 0, s,             \
+                                  static_cast<VT_*>(dst), static_cast<const VT_*>(src), P, swz);   \
+     } while (0)
           if (prec == Prec::C128) {
                if (T == 10) HQ_PERM_TILE(double, double2, 10);
                else if (T == 11) HQ_PERM_TILE(double, double2, 11);

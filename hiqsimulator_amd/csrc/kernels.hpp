@@ -68,6 +68,12 @@ void set_direct_block(int bs);
 int get_direct_block();
 void set_tile_high_target_wide(int pos);
 int get_tile_high_target_wide();
+// Tiled qubit permutation (n >= 12): low block-index bits reversed (0..16,
+// clamped to the grid) and threads per workgroup (256 or 512).
+void set_permute_swizzle(int bits);
+int get_permute_swizzle();
+void set_permute_block(int bs);
+int get_permute_block();
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

@@ -70,14 +70,15 @@ int hiqsim_run(hiqsim_t* h);
 /* Pass mode of hiqsim_run (no reference counterpart):
  *   HIQSIM_PASS_EACH (0): every cluster is one HBM pass at once, as the
  *     reference's Run() (SimulatorMPI.cpp:441-541);
- *   HIQSIM_PASS_DEFERRED (1, the default): on a state of 12 or more qubits a
- *     cluster of at most 5 qubits with its controls is queued; queued
- *     clusters run as tile passes, several clusters per HBM pass, planned
- *     with the LDS tile as the local set (as hiqsim_apply_tiled), once 64
- *     of them wait and before any call that reads the state, changes the
- *     qubit set or the qubit order, or synchronizes.  Results equal the
- *     reference's within floating-point rounding.
- * get: *mode, and *queued = clusters waiting (queued may be NULL). */
+ *   HIQSIM_PASS_DEFERRED (1, the default): on a state of 12 or more qubits
+ *     the gates of a cluster of at most 5 qubits with its controls are
+ *     queued (unfused); queued gates run as tile passes, several clusters
+ *     per HBM pass, planned with the LDS tile as the local set and fused per
+ *     tile op (as hiqsim_apply_tiled), once 512 of them wait (down to 256)
+ *     and before any call that reads the state, changes the qubit set or the
+ *     qubit order, or synchronizes.  Results equal the reference's within
+ *     floating-point rounding.
+ * get: *mode, and *queued = gates waiting (queued may be NULL). */
 enum hiqsim_pass_mode { HIQSIM_PASS_EACH = 0, HIQSIM_PASS_DEFERRED = 1 };
 int hiqsim_set_pass_mode(hiqsim_t* h, int mode);
 int hiqsim_get_pass_mode(hiqsim_t* h, int* mode, size_t* queued);

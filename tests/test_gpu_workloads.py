@@ -80,8 +80,10 @@ def test_drop_in_deferred_step_matches_eager_30q(engine_mod):
     g.init_random_state(7)
     bench.run_step(g, step)
     g.snapshot()
-    g.init_random_state(7)
+    # back to the identity order first: the random state is drawn per index,
+    # so both runs start from the same state only in the same qubit order
     g.permute_qubits(list(range(N30)))
+    g.init_random_state(7)
     g.set_pass_mode(1)
     g.set_timing(True)
     bench.run_step(g, step)
