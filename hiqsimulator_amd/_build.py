@@ -52,7 +52,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     cmds = []
-    for src in ("kernels.hip", "batch_kernels.hip", "engine.cpp", "batch.cpp", "serve.cpp",
+    for src in ("kernels.hip", "batch_kernels.hip", "noalloc.hip", "engine.cpp", "batch.cpp", "serve.cpp",
                 "capi.cpp", "sched.cpp"):
         obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]

@@ -8,6 +8,7 @@
 #include "batch.hpp"
 #include "engine.hpp"
 #include "sched.hpp"
+#include "noalloc.hpp"
 #include "serve.hpp"
 
 struct hiqsim_engine {
@@ -798,6 +799,17 @@ int hiqsim_sample(hiqsim_t* h, uint32_t shots, uint64_t seed, uint64_t* out)
           need(h, "handle");
           if (shots) need(out, "out");
           h->eng.sample(shots, seed, out);
+     });
+}
+
+int hiqsim_sample_noalloc(const void* d_state, int n_qubits, int precision, uint32_t shots,
+                          uint64_t seed, uint64_t* d_out, void* stream)
+{
+     return guard([&] {
+          if (precision != HIQSIM_C128 && precision != HIQSIM_C64)
+               throw std::runtime_error("precision must be HIQSIM_C128 or HIQSIM_C64");
+          hq::sample_noalloc(d_state, n_qubits, static_cast<hq::Prec>(precision), shots, seed, d_out,
+                             static_cast<hipStream_t>(stream));
      });
 }
 

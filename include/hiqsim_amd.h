@@ -171,6 +171,21 @@ int hiqsim_init_random_state(hiqsim_t* h, uint64_t seed);
  * probability exceeds u * norm.  No reference counterpart: the reference
  * samples by repeated MeasureQubits + collapse (SimulatorMPI.cpp:897-1008). */
 int hiqsim_sample(hiqsim_t* h, uint32_t shots, uint64_t seed, uint64_t* out);
+/* hiqsim_sample's sampler over a caller's device state, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, request loops that forbid
+ * hipMalloc).  d_state: 2^n_qubits amplitudes of the given precision in
+ * device memory of the current HIP device; d_out: `shots` uint64 in device
+ * memory, d_out[s] = sampled amplitude index, the same outcomes hiqsim_sample
+ * returns for that state and seed.  The work is enqueued on `stream` (a
+ * hipStream_t of the current device; NULL = the null stream) and nothing is
+ * synchronised.  Limits: n_qubits <= 35, shots <= 2^20 (larger values are
+ * rejected with an error).  Intermediates (2^15 block probabilities and two
+ * descriptors) live in one static device array of 256.5 KiB per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU (several streams or host threads) race. */
+int hiqsim_sample_noalloc(const void* d_state, int n_qubits, int precision, uint32_t shots,
+                          uint64_t seed, uint64_t* d_out, void* stream);
 
 /* get_expectation_value (_simulator_mpi.py:148-183; unbound in the
  * reference): sum_t coeff[t] <psi|P_t|psi>.  Bit i of xmask[t]/zmask[t] refers
