@@ -110,6 +110,22 @@ __device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
      return o;
 }
 
+// LDS slot of element r of a tile program's tile and of its staged op
+// matrices: the low four index bits XOR-ed with bits 4..7 (a bijection of
+// every aligned 256-element block).  LDS has 64 banks of 4 bytes, serving
+// a 64-lane 8-byte access in two 32-lane halves and a 16-byte access in four
+// 16-lane quarters.  In the identity layout the matrix-core operand loads
+// put lanes 2^rank or a 16-element matrix row apart on one bank (up to 16
+// accesses per bank in complex128); with this layout, modelled over the
+// ops of the 4096-circuit batch, the tile accesses need 0.79 (complex64) /
+// 0.54 (complex128) and the staged-matrix loads 0.33 / 0.11 of the bank
+// cycles.  It is linear over XOR, so for a group base b and an amplitude
+// offset o with disjoint bits, tslot(b + o) = tslot(b) ^ tslot(o).
+__device__ __forceinline__ uint32_t tslot(uint32_t r)
+{
+     return r ^ ((r >> 4) & 15u);
+}
+
 // The same pass on a 2^N-amplitude LDS tile with the matrix read from
 // global memory at a uniform address (scalar loads into SGPRs, shared by the
 // whole wave), two rows at a time: no LDS traffic for the matrix, and no
@@ -125,9 +141,9 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
      constexpr int D = 1 << K;
      constexpr int G = GN >= NTH ? GN / NTH : 1;  // groups per thread
      static_assert(GN < NTH || GN % NTH == 0, "groups must divide over the threads");
-     uint32_t off[D];
+     uint32_t off[D];  // LDS slots of the amplitude offsets (tslot)
 #pragma unroll
-     for (int j = 0; j < D; ++j) off[j] = amp_offset<K>(j, pos);
+     for (int j = 0; j < D; ++j) off[j] = tslot(amp_offset<K>(j, pos));
      if (GN < NTH && t0 >= static_cast<uint32_t>(GN)) return;
      uint32_t base[G];
      V v[G][D];
@@ -139,9 +155,9 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
                const uint32_t lo = b & ((1u << pos[l]) - 1u);
                b = ((b ^ lo) << 1) | lo;
           }
-          base[g] = b;
+          base[g] = tslot(b);
 #pragma unroll
-          for (int j = 0; j < D; ++j) v[g][j] = psi[b + off[j]];
+          for (int j = 0; j < D; ++j) v[g][j] = psi[base[g] ^ off[j]];
      }
 #pragma unroll 2
      for (int i = 0; i < D; ++i) {
@@ -160,7 +176,7 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
                V o;
                o.x = re;
                o.y = im;
-               psi[base[g] + off[i]] = o;
+               psi[base[g] ^ off[i]] = o;
           }
      }
 }
@@ -219,9 +235,11 @@ struct Mfma16<float> {
 };
 
 
-template <int K, typename R, int NWV, int NCB, int N = kTileBits>
-__device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5],
-                                            const int w, const uint32_t cb0)
+// mat[moff ..] is the op's matrix: in LDS at tslot(moff + index) when SW
+// (staged), else in global memory at moff + index.
+template <int K, typename R, int NWV, int NCB, bool SW, int N = kTileBits>
+__device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict__ mat, const uint32_t moff,
+                                            const uint8_t (&pos)[5], const int w, const uint32_t cb0)
 {
      // wave w of NWV takes column blocks cb0 + w + NWV * i of the NCB from cb0
      static_assert(K >= 3 && K <= 5, "MFMA pass needs 3 <= K <= 5");
@@ -243,7 +261,8 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
           const bool im_row = M::row_im(c);
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-               const V m = mat[o * D + 4 * t + q];
+               const uint32_t e = moff + static_cast<uint32_t>(o * D + 4 * t + q);
+               const V m = mat[SW ? tslot(e) : e];
                a[2 * t] = im_row ? m.y : m.x;       // times x.re
                a[2 * t + 1] = im_row ? m.x : -m.y;  // times x.im
           }
@@ -253,13 +272,13 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
 #pragma unroll
           for (int rb = 0; rb < RB; ++rb) load_a(rb, ahold[rb]);
      }
-     uint32_t in_off[T], out_off[RB][2];
+     uint32_t in_off[T], out_off[RB][2];  // LDS slots (tslot) of the amplitude offsets
 #pragma unroll
-     for (int t = 0; t < T; ++t) in_off[t] = amp_offset<K>(4 * t + q, pos);
+     for (int t = 0; t < T; ++t) in_off[t] = tslot(amp_offset<K>(4 * t + q, pos));
 #pragma unroll
      for (int rb = 0; rb < RB; ++rb) {
-          out_off[rb][0] = amp_offset<K>(rb * 8 + 2 * q, pos);
-          out_off[rb][1] = amp_offset<K>(rb * 8 + 2 * q + 1, pos);
+          out_off[rb][0] = tslot(amp_offset<K>(rb * 8 + 2 * q, pos));
+          out_off[rb][1] = tslot(amp_offset<K>(rb * 8 + 2 * q + 1, pos));
      }
      // The wave's column blocks in groups of HB: every amplitude of a group
      // is read before the group's first write.
@@ -276,9 +295,9 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
                     const uint32_t lo = b & ((1u << pos[l]) - 1u);
                     b = ((b ^ lo) << 1) | lo;
                }
-               base[i] = b;
+               base[i] = tslot(b);
 #pragma unroll
-               for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
+               for (int t = 0; t < T; ++t) x[i][t] = psi[base[i] ^ in_off[t]];
           }
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
@@ -294,8 +313,8 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
                     y0.y = acc[1];
                     y1.x = acc[2];
                     y1.y = acc[3];
-                    psi[base[i] + out_off[rb][0]] = y0;
-                    psi[base[i] + out_off[rb][1]] = y1;
+                    psi[base[i] ^ out_off[rb][0]] = y0;
+                    psi[base[i] ^ out_off[rb][1]] = y1;
                };
 #pragma unroll
                for (int rb = 0; rb < RB; ++rb) {
@@ -324,8 +343,8 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
 // of group c.  (For K = 3, D = 8 rows would fill half of each 16-row
 // instruction, so those ops keep the 2D x 2D form.)  The rounding error of
 // out.im grows from ~eps |M| |x| to ~2 eps (|Re M| + |Im M|)(|Re x| + |Im x|).
-template <int K, typename R, int NWV, int NCB, int N = kTileBits>
-__device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restrict__ mat,
+template <int K, typename R, int NWV, int NCB, bool SW, int N = kTileBits>
+__device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restrict__ mat, const uint32_t moff,
                                                const uint8_t (&pos)[5], const int w, const uint32_t cb0)
 {
      static_assert(K >= 4 && K <= 5, "three-product MFMA pass needs 4 <= K <= 5");
@@ -346,7 +365,8 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
           const int o = rb * 16 + c;
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-               const V m = mat[o * D + 4 * t + q];
+               const uint32_t e = moff + static_cast<uint32_t>(o * D + 4 * t + q);
+               const V m = mat[SW ? tslot(e) : e];
                a[3 * t] = m.x;
                a[3 * t + 1] = m.y;
                a[3 * t + 2] = m.x + m.y;
@@ -354,13 +374,13 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
      };
      R ahold[3 * T];
      if constexpr (kHoldA) load_a(0, ahold);
-     uint32_t in_off[T], out_off[RB][4];
+     uint32_t in_off[T], out_off[RB][4];  // LDS slots (tslot) of the amplitude offsets
 #pragma unroll
-     for (int t = 0; t < T; ++t) in_off[t] = amp_offset<K>(4 * t + q, pos);
+     for (int t = 0; t < T; ++t) in_off[t] = tslot(amp_offset<K>(4 * t + q, pos));
 #pragma unroll
      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) out_off[rb][j] = amp_offset<K>(rb * 16 + M::out_row(q, j), pos);
+          for (int j = 0; j < 4; ++j) out_off[rb][j] = tslot(amp_offset<K>(rb * 16 + M::out_row(q, j), pos));
      constexpr int HB = CB < 2 ? CB : 2;
 #pragma unroll
      for (int i0 = 0; i0 < CB; i0 += HB) {
@@ -374,9 +394,9 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
                     const uint32_t lo = b & ((1u << pos[l]) - 1u);
                     b = ((b ^ lo) << 1) | lo;
                }
-               base[i] = b;
+               base[i] = tslot(b);
 #pragma unroll
-               for (int t = 0; t < T; ++t) x[i][t] = psi[b + in_off[t]];
+               for (int t = 0; t < T; ++t) x[i][t] = psi[base[i] ^ in_off[t]];
           }
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
@@ -393,7 +413,7 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
                          V y;
                          y.x = p1[j] - p2[j];
                          y.y = p3[j] - p1[j] - p2[j];
-                         psi[base[i] + out_off[rb][j]] = y;
+                         psi[base[i] ^ out_off[rb][j]] = y;
                     }
                };
 #pragma unroll
@@ -411,12 +431,13 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
      }
 }
 
-template <int K, typename R, int NT, int N = kTileBits>
-__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint8_t (&pos)[5])
+template <int K, typename R, int NT, bool SW, int N = kTileBits>
+__device__ __forceinline__ void wg_pass_mfma(VT<R>* psi, const VT<R>* __restrict__ mat, const uint32_t moff,
+                                             const uint8_t (&pos)[5])
 {
      const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-     if constexpr (K >= 4) mfma_blocks_3m<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos, w, 0);
-     else mfma_blocks<K, R, NT / 64, (1 << (N - K)) / 16, N>(psi, mat, pos, w, 0);
+     if constexpr (K >= 4) mfma_blocks_3m<K, R, NT / 64, (1 << (N - K)) / 16, SW, N>(psi, mat, moff, pos, w, 0);
+     else mfma_blocks<K, R, NT / 64, (1 << (N - K)) / 16, SW, N>(psi, mat, moff, pos, w, 0);
 }
 
 // |0..0> followed by every pass of circuit c, on psi.  Each pass's matrix is
@@ -548,13 +569,13 @@ struct TileIo {
      {
           const int lane = threadIdx.x & 63, w = wave();
 #pragma unroll
-          for (int i = 0; i < PER; ++i) tile[(w + i * NW) * 64 + lane] = r[i];
+          for (int i = 0; i < PER; ++i) tile[tslot(static_cast<uint32_t>((w + i * NW) * 64 + lane))] = r[i];
      }
      static __device__ __forceinline__ void from_lds(const V* tile, V (&r)[PER])
      {
           const int lane = threadIdx.x & 63, w = wave();
 #pragma unroll
-          for (int i = 0; i < PER; ++i) r[i] = tile[(w + i * NW) * 64 + lane];
+          for (int i = 0; i < PER; ++i) r[i] = tile[tslot(static_cast<uint32_t>((w + i * NW) * 64 + lane))];
      }
 };
 
@@ -610,7 +631,7 @@ struct MatStage {
 #pragma unroll
           for (int i = 0; i < PER; ++i) {
                const int e = static_cast<int>(threadIdx.x) + i * NT;
-               if (e < n) smat[e] = r[i];
+               if (e < n) smat[tslot(static_cast<uint32_t>(e))] = r[i];
           }
      }
 };
@@ -653,16 +674,16 @@ __device__ __forceinline__ void tile_ops(VT<R>* tile, const VT<R>* smat, const i
           if constexpr (MK <= 5) {
                if (o.k >= MK) {
                     if (in_lds) {
-                         const V* const m = smat + soff;
-                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, m, o.rpos);
-                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, m, o.rpos);
-                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, m, o.rpos);
+                         const uint32_t mo = static_cast<uint32_t>(soff);
+                         if (o.k == 3) wg_pass_mfma<3, R, NT, true>(tile, smat, mo, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT, true>(tile, smat, mo, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT, true>(tile, smat, mo, o.rpos);
                     }
                     else {
                          const V* const m = mats + o.mat_off;
-                         if (o.k == 3) wg_pass_mfma<3, R, NT>(tile, m, o.rpos);
-                         else if (o.k == 4) wg_pass_mfma<4, R, NT>(tile, m, o.rpos);
-                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT>(tile, m, o.rpos);
+                         if (o.k == 3) wg_pass_mfma<3, R, NT, false>(tile, m, 0, o.rpos);
+                         else if (o.k == 4) wg_pass_mfma<4, R, NT, false>(tile, m, 0, o.rpos);
+                         else if constexpr (KMAX >= 5) wg_pass_mfma<5, R, NT, false>(tile, m, 0, o.rpos);
                     }
                     soff += sz;
                     continue;
