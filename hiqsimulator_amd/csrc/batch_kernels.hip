@@ -126,6 +126,45 @@ __device__ __forceinline__ uint32_t tslot(uint32_t r)
      return r ^ ((r >> 4) & 15u);
 }
 
+// Element of an LDS array at a byte offset (offsets are combined with XOR in
+// bytes, so no per-access shift of an element index is needed).
+template <typename V>
+__device__ __forceinline__ V& lds_at(V* base, uint32_t bytes)
+{
+     return *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + bytes);
+}
+
+// The compiler folds (a << s) ^ (b << s) back into (a ^ b) << s, which puts
+// a shift in front of every LDS access; an empty asm statement makes a
+// byte offset opaque to it (no instruction is emitted).
+__device__ __forceinline__ uint32_t opaque_v(uint32_t x)
+{
+     asm("" : "+v"(x));
+     return x;
+}
+__device__ __forceinline__ uint32_t opaque_s(uint32_t x)
+{
+     asm("" : "+s"(x));
+     return x;
+}
+
+// Group index v spread over the non-target positions: a zero bit inserted
+// at each of pos[0] < ... < pos[K-1].  Bitwise, so index parts with disjoint
+// bits spread to disjoint bits: the addresses below split into a per-lane
+// part, computed once per op, and a part uniform across the wave, which
+// the compiler keeps in scalar registers (the scalar unit computes it, the
+// vector unit only XORs it in).
+template <int K>
+__device__ __forceinline__ uint32_t insert_targets(uint32_t v, const uint8_t (&pos)[5])
+{
+#pragma unroll
+     for (int l = 0; l < K; ++l) {
+          const uint32_t lo = v & ((1u << pos[l]) - 1u);
+          v = ((v ^ lo) << 1) | lo;
+     }
+     return v;
+}
+
 // The same pass on a 2^N-amplitude LDS tile with the matrix read from
 // global memory at a uniform address (scalar loads into SGPRs, shared by the
 // whole wave), two rows at a time: no LDS traffic for the matrix, and no
@@ -141,23 +180,22 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
      constexpr int D = 1 << K;
      constexpr int G = GN >= NTH ? GN / NTH : 1;  // groups per thread
      static_assert(GN < NTH || GN % NTH == 0, "groups must divide over the threads");
-     uint32_t off[D];  // LDS slots of the amplitude offsets (tslot)
+     static_assert((NTH & (NTH - 1)) == 0, "NTH must be a power of two");
+     constexpr uint32_t SH = sizeof(V) == 16 ? 4 : 3;  // byte offsets below
+     uint32_t off[D];  // LDS slots of the amplitude offsets (tslot; uniform)
 #pragma unroll
-     for (int j = 0; j < D; ++j) off[j] = tslot(amp_offset<K>(j, pos));
+     for (int j = 0; j < D; ++j) off[j] = opaque_s(tslot(amp_offset<K>(j, pos)) << SH);
      if (GN < NTH && t0 >= static_cast<uint32_t>(GN)) return;
+     // group g0 + NTH * g + t0 with g0 a multiple of NTH and t0 < NTH: the
+     // lane part t0 and the uniform part have disjoint bits
+     const uint32_t vt = opaque_v(tslot(insert_targets<K>(t0, pos)) << SH);
      uint32_t base[G];
      V v[G][D];
 #pragma unroll
      for (int g = 0; g < G; ++g) {
-          uint32_t b = g0 + t0 + static_cast<uint32_t>(g) * NTH;
+          base[g] = vt ^ opaque_s(tslot(insert_targets<K>(g0 + static_cast<uint32_t>(g) * NTH, pos)) << SH);
 #pragma unroll
-          for (int l = 0; l < K; ++l) {
-               const uint32_t lo = b & ((1u << pos[l]) - 1u);
-               b = ((b ^ lo) << 1) | lo;
-          }
-          base[g] = tslot(b);
-#pragma unroll
-          for (int j = 0; j < D; ++j) v[g][j] = psi[base[g] ^ off[j]];
+          for (int j = 0; j < D; ++j) v[g][j] = lds_at(psi, base[g] ^ off[j]);
      }
 #pragma unroll 2
      for (int i = 0; i < D; ++i) {
@@ -176,7 +214,7 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
                V o;
                o.x = re;
                o.y = im;
-               psi[base[g] ^ off[i]] = o;
+               lds_at(psi, base[g] ^ off[i]) = o;
           }
      }
 }
@@ -272,35 +310,38 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
 #pragma unroll
           for (int rb = 0; rb < RB; ++rb) load_a(rb, ahold[rb]);
      }
-     uint32_t in_off[T], out_off[RB][2];  // LDS slots (tslot) of the amplitude offsets
+     // LDS slots split into lane and uniform parts (disjoint index bits):
+     // group c of a column block, input amplitude 4t + q, output amplitudes
+     // 8 rb + 2q + h
+     constexpr uint32_t SH = sizeof(V) == 16 ? 4 : 3;  // byte offsets below
+     const uint32_t vc = tslot(insert_targets<K>(static_cast<uint32_t>(c), pos));
+     const uint32_t vin = opaque_v((vc ^ tslot(amp_offset<K>(q, pos))) << SH);
+     const uint32_t vout = opaque_v((vc ^ tslot(amp_offset<K>(2 * q, pos))) << SH);
+     uint32_t in_off[T], out_off[RB][2];  // uniform
 #pragma unroll
-     for (int t = 0; t < T; ++t) in_off[t] = tslot(amp_offset<K>(4 * t + q, pos));
+     for (int t = 0; t < T; ++t) in_off[t] = opaque_s(tslot(amp_offset<K>(4 * t, pos)) << SH);
 #pragma unroll
      for (int rb = 0; rb < RB; ++rb) {
-          out_off[rb][0] = tslot(amp_offset<K>(rb * 8 + 2 * q, pos));
-          out_off[rb][1] = tslot(amp_offset<K>(rb * 8 + 2 * q + 1, pos));
+          out_off[rb][0] = opaque_s(tslot(amp_offset<K>(rb * 8, pos)) << SH);
+          out_off[rb][1] = opaque_s(tslot(amp_offset<K>(rb * 8 + 1, pos)) << SH);
      }
      // The wave's column blocks in groups of HB: every amplitude of a group
      // is read before the group's first write.
      constexpr int HB = CB < 2 ? CB : 2;
 #pragma unroll
      for (int i0 = 0; i0 < CB; i0 += HB) {
-          uint32_t base[HB];
+          uint32_t base[HB];  // uniform part of the column block's groups
           V x[HB][T];
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
-               uint32_t b = (cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16 + c;
+               base[i] = opaque_s(tslot(insert_targets<K>((cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16, pos)) << SH);
+               const uint32_t bi = vin ^ base[i];
 #pragma unroll
-               for (int l = 0; l < K; ++l) {
-                    const uint32_t lo = b & ((1u << pos[l]) - 1u);
-                    b = ((b ^ lo) << 1) | lo;
-               }
-               base[i] = tslot(b);
-#pragma unroll
-               for (int t = 0; t < T; ++t) x[i][t] = psi[base[i] ^ in_off[t]];
+               for (int t = 0; t < T; ++t) x[i][t] = lds_at(psi, bi ^ in_off[t]);
           }
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
+               const uint32_t bo = vout ^ base[i];
                auto block = [&](const R (&a)[2 * T], int rb) {
                     acc_t acc = {0, 0, 0, 0};
 #pragma unroll
@@ -313,8 +354,8 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
                     y0.y = acc[1];
                     y1.x = acc[2];
                     y1.y = acc[3];
-                    psi[base[i] ^ out_off[rb][0]] = y0;
-                    psi[base[i] ^ out_off[rb][1]] = y1;
+                    lds_at(psi, bo ^ out_off[rb][0]) = y0;
+                    lds_at(psi, bo ^ out_off[rb][1]) = y1;
                };
 #pragma unroll
                for (int rb = 0; rb < RB; ++rb) {
@@ -374,32 +415,36 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
      };
      R ahold[3 * T];
      if constexpr (kHoldA) load_a(0, ahold);
-     uint32_t in_off[T], out_off[RB][4];  // LDS slots (tslot) of the amplitude offsets
+     // LDS slots split into lane and uniform parts (disjoint index bits):
+     // group c of a column block, input amplitude 4t + q, output amplitude
+     // 16 rb + out_row(q, j) = 16 rb + out_row(q, 0) ^ out_row(0, j)
+     constexpr uint32_t SH = sizeof(V) == 16 ? 4 : 3;  // byte offsets below
+     const uint32_t vc = tslot(insert_targets<K>(static_cast<uint32_t>(c), pos));
+     const uint32_t vin = opaque_v((vc ^ tslot(amp_offset<K>(q, pos))) << SH);
+     const uint32_t vout = opaque_v((vc ^ tslot(amp_offset<K>(M::out_row(q, 0), pos))) << SH);
+     uint32_t in_off[T], out_off[RB][4];  // uniform
 #pragma unroll
-     for (int t = 0; t < T; ++t) in_off[t] = tslot(amp_offset<K>(4 * t + q, pos));
+     for (int t = 0; t < T; ++t) in_off[t] = opaque_s(tslot(amp_offset<K>(4 * t, pos)) << SH);
 #pragma unroll
      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) out_off[rb][j] = tslot(amp_offset<K>(rb * 16 + M::out_row(q, j), pos));
+          for (int j = 0; j < 4; ++j)
+               out_off[rb][j] = opaque_s(tslot(amp_offset<K>(rb * 16 + M::out_row(0, j), pos)) << SH);
      constexpr int HB = CB < 2 ? CB : 2;
 #pragma unroll
      for (int i0 = 0; i0 < CB; i0 += HB) {
-          uint32_t base[HB];
+          uint32_t base[HB];  // uniform part of the column block's groups
           V x[HB][T];
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
-               uint32_t b = (cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16 + c;
+               base[i] = opaque_s(tslot(insert_targets<K>((cb0 + static_cast<uint32_t>(w + (i0 + i) * NWV)) * 16, pos)) << SH);
+               const uint32_t bi = vin ^ base[i];
 #pragma unroll
-               for (int l = 0; l < K; ++l) {
-                    const uint32_t lo = b & ((1u << pos[l]) - 1u);
-                    b = ((b ^ lo) << 1) | lo;
-               }
-               base[i] = tslot(b);
-#pragma unroll
-               for (int t = 0; t < T; ++t) x[i][t] = psi[base[i] ^ in_off[t]];
+               for (int t = 0; t < T; ++t) x[i][t] = lds_at(psi, bi ^ in_off[t]);
           }
 #pragma unroll
           for (int i = 0; i < HB; ++i) {
+               const uint32_t bo = vout ^ base[i];
                auto block = [&](const R (&a)[3 * T], int rb) {
                     acc_t p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
 #pragma unroll
@@ -413,7 +458,7 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
                          V y;
                          y.x = p1[j] - p2[j];
                          y.y = p3[j] - p1[j] - p2[j];
-                         psi[base[i] ^ out_off[rb][j]] = y;
+                         lds_at(psi, bo ^ out_off[rb][j]) = y;
                     }
                };
 #pragma unroll
