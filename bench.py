@@ -16,7 +16,8 @@ cpu_baseline = the reference's own AVX2 kernels (oracle/_ref, OpenMP with
                OMP_PROC_BIND=spread, OMP_PLACES=cores) in a child process
                started before this one touches the GPU: per k = 2..5 the
                median of 5 passes on the 2^n-amplitude complex128 state, and
-               the headline circuit's fused passes end to end.
+               the first 36 of the headline circuit's fused passes (about
+               11 s), the whole circuit's time projected from them.
 Also reported: the step at every fusion width (circuit_k_sweep), the drop-in
 flow (run() per cluster with deferred tile passes), tile programs, the
 QFT-like precision round trip, the 4096-circuit batch and the serving
@@ -201,11 +202,13 @@ def cpu_baseline_child(args):
     (oracle/_ref, built with its release flags) dispatched as its Run() does.
     * per k = 2..5: one warm pass, then the median of --cpu-reps passes at
       random placements on the 2^n-amplitude complex128 state;
-    * the whole headline circuit: every fused pass of the step (the
-      reference's fusion, then one kernel pass per cluster); the step's
+    * the headline circuit's fused passes (the reference's fusion, then one
+      kernel pass per cluster), a bounded prefix of --cpu-circuit-passes of
+      them, with the whole circuit's time projected from it; the step's
       qubit permutations have no counterpart on the reference's one-node path."""
     import oracle
     from hiqsimulator_amd.circuits import haar_unitary
+    info = cpu_info()  # before OpenMP binds this thread to one core
     n = args.qubits
     if not oracle.RefKernels.available("intrin"):
         return {"value": None, "unit": "GB/s", "error": "oracle/_ref not built"}
@@ -234,7 +237,8 @@ def cpu_baseline_child(args):
         t0 = time.perf_counter()
         gates, step, _ = build_step(n, args.layers, args.k, args.seed, args.perms, args.planner)
         plan_s = time.perf_counter() - t0
-        passes = [payload for kind, payload in step if kind == "pass"]
+        all_passes = [payload for kind, payload in step if kind == "pass"]
+        passes = all_passes[:max(1, args.cpu_circuit_passes)]
         ref.fill(h, 2.0 ** (-n / 2))
         t0 = time.perf_counter()
         for cl in passes:
@@ -250,15 +254,17 @@ def cpu_baseline_child(args):
             M, ids, _, diag = f.perform(cap_qubits=5)
             ref.apply(h, M, ids, 0, diag)
         el = time.perf_counter() - t0
-        circuit = {"seconds": round(el, 2), "fused_passes": len(passes), "gates": len(gates),
+        circuit = {"seconds": round(el, 2), "fused_passes_timed": len(passes),
+                   "fused_passes": len(all_passes), "gates": len(gates),
+                   "projected_circuit_seconds": round(el * len(all_passes) / len(passes), 2),
                    "fusion_k": args.k, "plan_seconds": round(plan_s, 2),
                    "gbs": round(32.0 * 2 ** n * len(passes) / el / 1e9, 2),
-                   "note": "the headline step's fused passes (reference fusion + kernels); its %d qubit "
-                           "permutations have no counterpart on the reference's one-node path"
-                           % sum(1 for kind, _ in step if kind == "perm")}
+                   "note": "the first %d of the headline step's %d fused passes (reference fusion + "
+                           "kernels), whole circuit projected; its %d qubit permutations have no "
+                           "counterpart on the reference's one-node path"
+                           % (len(passes), len(all_passes), sum(1 for kind, _ in step if kind == "perm"))}
         print("cpu circuit: %s" % circuit, file=sys.stderr, flush=True)
     ref.free_state(h)
-    info = cpu_info()
     return {"value": per_k[str(args.k)]["gbs"], "unit": "GB/s", "cores": ref.num_threads(),
             "kind": "reference",
             "sample": "reference AVX2 kernels (oracle/_ref, -O3 -ffast-math -mavx2 -mfma, OpenMP) on a "
@@ -281,7 +287,8 @@ def cpu_baseline(args):
         env["OMP_NUM_THREADS"] = str(min(phys, len(os.sched_getaffinity(0))))
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--qubits", str(args.qubits),
            "--layers", str(args.layers), "--k", str(args.k), "--seed", str(args.seed), "--perms",
-           str(args.perms), "--planner", args.planner, "--cpu-reps", str(args.cpu_reps)]
+           str(args.perms), "--planner", args.planner, "--cpu-reps", str(args.cpu_reps),
+           "--cpu-circuit-passes", str(args.cpu_circuit_passes)]
     if args.no_cpu_circuit:
         cmd.append("--no-cpu-circuit")
     r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True)
@@ -568,6 +575,8 @@ def main():
     ap.add_argument("--cpu-baseline-only", action="store_true",
                     help="run only the CPU baseline and print its JSON (the child process bench.py starts)")
     ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU passes per k (median)")
+    ap.add_argument("--cpu-circuit-passes", type=int, default=36,
+                    help="fused passes of the headline circuit timed on the CPU (about 11 s at 30 qubits)")
     ap.add_argument("--no-cpu-circuit", action="store_true",
                     help="skip the whole-circuit CPU timing (about a minute at 30 qubits)")
     ap.add_argument("--no-drop-in", action="store_true")

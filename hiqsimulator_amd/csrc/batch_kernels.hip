@@ -688,16 +688,17 @@ struct MatStage {
 constexpr int kTileVector = 0, kTileMatrix = 1;
 constexpr int tile_mfma_min_k(int mode) { return mode == kTileVector ? 6 : 3; }
 
-// Waves per SIMD the tile kernels are built for: two 512-thread workgroups
-// per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
-// VGPRs), four for complex64 (40 KiB; at most 64 VGPRs).  Programs with
+// Waves per SIMD the tile kernels are built for.  512-thread workgroups:
+// two per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
+// VGPRs), four for complex64 (40 KiB; at most 64 VGPRs); programs with
 // 5-qubit ops get more registers (complex128 unconstrained, complex64 two
-// workgroups per CU): their 5-qubit passes need them.
-template <typename R, int KMAX>
+// workgroups per CU): their 5-qubit passes need them.  256-thread
+// workgroups: the same workgroups per CU, so half the waves.
+template <typename R, int KMAX, int NT>
 constexpr int tile_waves_per_eu()
 {
-     if (sizeof(R) == 8) return KMAX >= 5 ? 1 : 4;
-     return KMAX >= 5 ? 4 : 8;
+     const int w512 = sizeof(R) == 8 ? (KMAX >= 5 ? 1 : 4) : (KMAX >= 5 ? 4 : 8);
+     return NT >= 512 ? w512 : (w512 > 1 ? w512 / 2 : 1);
 }
 
 // ops[0..nops) on the tile in LDS, a workgroup barrier before each.  The
@@ -779,7 +780,7 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
 // Tile pass over one state: workgroup b runs the program on tile
 // swizzle(b).
 template <typename R, int KMAX, int NT, int MODE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX, NT>())))
     k_tile_pass(VT<R>* __restrict__ psi, const VT<R>* __restrict__ mats,
                 const TileOp* __restrict__ ops, const int nops, const TileHigh h, const int swz,
                 const uint64_t nblocks)
@@ -794,7 +795,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_wa
 // Tile passes of many states: block b belongs to the item whose blocks
 // contain it; the item's tiles are visited in swizzled order.
 template <typename R, int KMAX, int NT, int MODE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX>())))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<R, KMAX, NT>())))
     k_batch_tprog(VT<R>* __restrict__ arena, const VT<R>* __restrict__ mats,
                   const BatchTileProg* __restrict__ items, const int nitems,
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
@@ -1087,13 +1088,15 @@ hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const Ba
      const dim3 grid = grid_2d(total_blocks);
      const int swz = get_block_swizzle();
      const int mode = get_tile_mfma();
-#define HQ_BTP(RR, KM, MODE)                                                                       \
-     hipLaunchKernelGGL((k_batch_tprog<RR, KM, 512, MODE>), grid, dim3(512), 0, s,                 \
+     const bool narrow = tile_threads_for(prec) == 256;
+#define HQ_BTP(RR, KM, NT, MODE)                                                                   \
+     hipLaunchKernelGGL((k_batch_tprog<RR, KM, NT, MODE>), grid, dim3(NT), 0, s,                   \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
                         nitems, ops, total_blocks, swz)
-#define HQ_BTP2(RR, KM)                                  \
-     if (mode == kTileMatrix) HQ_BTP(RR, KM, kTileMatrix); \
-     else HQ_BTP(RR, KM, kTileVector)
+#define HQ_BTP2(RR, KM)                                                \
+     if (mode == kTileMatrix && narrow) HQ_BTP(RR, KM, 256, kTileMatrix); \
+     else if (mode == kTileMatrix) HQ_BTP(RR, KM, 512, kTileMatrix);      \
+     else HQ_BTP(RR, KM, 512, kTileVector)
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_BTP2(double, 5);
           else HQ_BTP2(double, 4);
@@ -1120,13 +1123,15 @@ hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& 
      const int swz = (bits > 0 && bits <= 16 && blocks % (1ull << bits) == 0) ? bits : 0;
      const dim3 grid = grid_2d(blocks);
      const int mode = get_tile_mfma();
-#define HQ_TP(RR, KM, MODE)                                                                        \
-     hipLaunchKernelGGL((k_tile_pass<RR, KM, 512, MODE>), grid, dim3(512), 0, s,                   \
+     const bool narrow = tile_threads_for(prec) == 256;
+#define HQ_TP(RR, KM, NT, MODE)                                                                    \
+     hipLaunchKernelGGL((k_tile_pass<RR, KM, NT, MODE>), grid, dim3(NT), 0, s,                     \
                         static_cast<VT<RR>*>(psi), static_cast<const VT<RR>*>(mats), ops, nops, h, \
                         swz, blocks)
-#define HQ_TP2(RR, KM)                                  \
-     if (mode == kTileMatrix) HQ_TP(RR, KM, kTileMatrix); \
-     else HQ_TP(RR, KM, kTileVector)
+#define HQ_TP2(RR, KM)                                                \
+     if (mode == kTileMatrix && narrow) HQ_TP(RR, KM, 256, kTileMatrix); \
+     else if (mode == kTileMatrix) HQ_TP(RR, KM, 512, kTileMatrix);      \
+     else HQ_TP(RR, KM, 512, kTileVector)
      if (prec == Prec::C128) {
           if (kmax >= 5) HQ_TP2(double, 5);
           else HQ_TP2(double, 4);

@@ -651,6 +651,11 @@ int hiqsim_set_tuning(const char* name, int64_t value)
                if (value < 0 || value > 16) throw std::runtime_error("permute_swizzle must be 0..16");
                hq::set_permute_swizzle(static_cast<int>(value));
           }
+          else if (n == "tile_threads") {
+               if (value != 0 && value != 256 && value != 512)
+                    throw std::runtime_error("tile_threads must be 0 (per precision), 256 or 512");
+               hq::set_tile_threads(static_cast<int>(value));
+          }
           else if (n == "permute_block") {
                if (value != 256 && value != 512) throw std::runtime_error("permute_block must be 256 or 512");
                hq::set_permute_block(static_cast<int>(value));
@@ -675,6 +680,7 @@ int hiqsim_get_tuning(const char* name, int64_t* value)
           else if (n == "tile_high_target_wide") *value = hq::get_tile_high_target_wide();
           else if (n == "permute_swizzle") *value = hq::get_permute_swizzle();
           else if (n == "permute_block") *value = hq::get_permute_block();
+          else if (n == "tile_threads") *value = hq::get_tile_threads();
           else throw std::runtime_error("unknown tuning parameter '" + n + "'");
      });
 }

@@ -616,6 +616,13 @@ std::atomic<int> g_direct_block{1024};
 // workgroup (256 or 512).
 std::atomic<int> g_permute_swizzle{0};
 std::atomic<int> g_permute_block{256};
+// Threads per workgroup of the tile-program kernels on the matrix cores: 0
+// picks per precision.  Measured on the 4096-circuit batch: complex64 359
+// ms with 256 threads against 376 with 512 (the per-op lane setup is shared
+// by four column blocks per wave instead of two); complex128 693 against
+// 680 ms, and the 30-qubit tile pass 8.07 against 7.85 ms (two workgroups
+// per CU leave too few waves at 256).
+std::atomic<int> g_tile_threads{0};
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
 // configured width when the grid holds whole runs of 2^bits blocks, else
@@ -820,6 +827,13 @@ void set_permute_swizzle(int bits) { g_permute_swizzle.store(bits < 0 ? 0 : (bit
 int get_permute_swizzle() { return g_permute_swizzle.load(kRelaxed); }
 void set_permute_block(int bs) { g_permute_block.store(bs == 512 ? 512 : 256, kRelaxed); }
 int get_permute_block() { return g_permute_block.load(kRelaxed); }
+void set_tile_threads(int nt) { g_tile_threads.store(nt == 256 || nt == 512 ? nt : 0, kRelaxed); }
+int get_tile_threads() { return g_tile_threads.load(kRelaxed); }
+int tile_threads_for(Prec p)
+{
+     const int nt = get_tile_threads();
+     return nt ? nt : (p == Prec::C64 ? 256 : 512);
+}
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,
                         hipStream_t s)

@@ -74,6 +74,11 @@ void set_permute_swizzle(int bits);
 int get_permute_swizzle();
 void set_permute_block(int bs);
 int get_permute_block();
+// Threads per workgroup of the tile-program kernels with matrix-core ops:
+// 256, 512 or 0 (the default: 256 for complex64, 512 for complex128).
+void set_tile_threads(int nt);
+int get_tile_threads();
+int tile_threads_for(Prec p);
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).
