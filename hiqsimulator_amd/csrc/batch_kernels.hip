@@ -373,17 +373,20 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
 }
 
 // The same pass for 4 <= K <= 5 with three real D x D products instead of
-// the 2D x 2D real form (Gauss's three-multiplication complex product):
-//   P1 = Re M . Re x,  P2 = Im M . Im x,  P3 = (Re M + Im M) . (Re x + Im x),
-//   out.re = P1 - P2,  out.im = P3 - P1 - P2,
-// 3 D^2 / 64 instructions per 16 groups instead of 4 D^2 / 64.  Here the
+// the 2D x 2D real form (a three-multiplication complex product in the
+// manner of Gauss's; with M = A + iB and x = u + iv):
+//   Q1 = A . (u + v),  Q2 = (A + B) . v,  Q3 = (B - A) . u,
+//   out.re = Q1 - Q2 = A u - B v,  out.im = Q1 + Q3 = A v + B u,
+// 3 D^2 / 64 instructions per 16 groups instead of 4 D^2 / 64, and two
+// vector instructions per output amplitude (the form P1 = A u, P2 = B v,
+// P3 = (A + B)(u + v), out.im = P3 - P1 - P2 takes three).  Here the
 // real rows of a 16-row block are the amplitudes 16 * rb + r in natural
 // order; instruction s takes input amplitude 4 * s + q from lane (q, c), so
 // one LDS load of a complex amplitude feeds three instructions, and lane
 // (q, c)'s accumulator element j is output amplitude 16 * rb + out_row(q, j)
 // of group c.  (For K = 3, D = 8 rows would fill half of each 16-row
-// instruction, so those ops keep the 2D x 2D form.)  The rounding error of
-// out.im grows from ~eps |M| |x| to ~2 eps (|Re M| + |Im M|)(|Re x| + |Im x|).
+// instruction, so those ops keep the 2D x 2D form.)  The rounding error
+// grows from ~eps |M| |x| to ~2 eps (|A| + |B|)(|u| + |v|).
 template <int K, typename R, int NWV, int NCB, bool SW, int N = kTileBits>
 __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restrict__ mat, const uint32_t moff,
                                                const uint8_t (&pos)[5], const int w, const uint32_t cb0)
@@ -408,9 +411,9 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
           for (int t = 0; t < T; ++t) {
                const uint32_t e = moff + static_cast<uint32_t>(o * D + 4 * t + q);
                const V m = mat[SW ? tslot(e) : e];
-               a[3 * t] = m.x;
-               a[3 * t + 1] = m.y;
-               a[3 * t + 2] = m.x + m.y;
+               a[3 * t] = m.x;               // A, times u + v
+               a[3 * t + 1] = m.x + m.y;     // A + B, times v
+               a[3 * t + 2] = m.y - m.x;     // B - A, times u
           }
      };
      R ahold[3 * T];
@@ -449,15 +452,15 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
                     acc_t p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
 #pragma unroll
                     for (int t = 0; t < T; ++t) {
-                         p1 = M::step(a[3 * t], x[i][t].x, p1);
+                         p1 = M::step(a[3 * t], x[i][t].x + x[i][t].y, p1);
                          p2 = M::step(a[3 * t + 1], x[i][t].y, p2);
-                         p3 = M::step(a[3 * t + 2], x[i][t].x + x[i][t].y, p3);
+                         p3 = M::step(a[3 * t + 2], x[i][t].x, p3);
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                          V y;
                          y.x = p1[j] - p2[j];
-                         y.y = p3[j] - p1[j] - p2[j];
+                         y.y = p1[j] + p3[j];
                          lds_at(psi, bo ^ out_off[rb][j]) = y;
                     }
                };
