@@ -206,14 +206,17 @@ def test_direct_block_sizes(engine_mod, oracle_mod, block):
         mod.set_tuning("tile_high_target_wide", old[1])
 
 
-@pytest.mark.parametrize("swizzle,high", [(0, 20), (4, 20), (8, 64), (10, 64), (16, 0)])
+@pytest.mark.parametrize("swizzle,high", [(0, 20), (4, 20), (8, 64), (10, 64), (16, 0), (0, 0)])
 def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
     """Block-order swizzle and the tile/direct threshold change only the
-    order and form of the work: results equal the oracle at every setting."""
+    order and form of the work: results equal the oracle at every setting.
+    The threshold is set for 4-qubit passes too (tile_high_target_wide), so
+    with high = 0 the 4-qubit passes at low placements take the direct form,
+    swizzled and not."""
     import hiqsimulator_amd
     mod = hiqsimulator_amd._cppsim_amd
     old = (mod.get_tuning("block_swizzle"), mod.get_tuning("tile_high_target"),
-           mod.get_tuning("block_swizzle_wide"))
+           mod.get_tuning("block_swizzle_wide"), mod.get_tuning("tile_high_target_wide"))
     n = 22
     rng = np.random.default_rng(7 + swizzle)
     st = rand_state(rng, n)
@@ -221,6 +224,7 @@ def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
         mod.set_tuning("block_swizzle", swizzle)
         mod.set_tuning("block_swizzle_wide", swizzle)
         mod.set_tuning("tile_high_target", high)
+        mod.set_tuning("tile_high_target_wide", high)
         print("swizzle", swizzle, "high", high, flush=True)
         for pos in ([0, 1, 2, 3], [18, 19, 20, 21], [2, 9, 15, 21], [20], [0, 5, 10, 15, 20]):
             k = len(pos)
@@ -235,3 +239,4 @@ def test_block_order_and_form_knobs(engine_mod, oracle_mod, swizzle, high):
         mod.set_tuning("block_swizzle", old[0])
         mod.set_tuning("tile_high_target", old[1])
         mod.set_tuning("block_swizzle_wide", old[2])
+        mod.set_tuning("tile_high_target_wide", old[3])
