@@ -386,14 +386,13 @@ Batch::HostPlan Batch::plan_host(int workgroups)
      hp.mats.resize(std::max<uint64_t>(mat_entries, 1) * ab);
      std::vector<char>& h_mats = hp.mats;
      parallel_for(nc, [&](size_t c) {
-          std::vector<std::pair<const std::vector<cplx>*, uint64_t>> all;
-          for (size_t p = 0; p < plan[c].size(); ++p) all.push_back({&plan[c][p].m, mat_off[c][p]});
           size_t q = 0;
           for (const auto& t: tiles[c])
-               for (const auto& op: t.ops) all.push_back({&op.m, tmat_off[c][q++]});
-          for (const auto& mo: all) {
-               const auto& m = *mo.first;
-               const uint64_t moff = mo.second;
+               for (const auto& op: t.ops)
+                    put_tile_matrix(h_mats.data() + tmat_off[c][q++] * ab, op.m.data(), op.k, prec_);
+          for (size_t p = 0; p < plan[c].size(); ++p) {
+               const auto& m = plan[c][p].m;
+               const uint64_t moff = mat_off[c][p];
                if (prec_ == Prec::C128) {
                     double* d = reinterpret_cast<double*>(h_mats.data()) + 2 * moff;
                     for (size_t i = 0; i < m.size(); ++i) {

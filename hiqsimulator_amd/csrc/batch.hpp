@@ -157,11 +157,46 @@ hipError_t launch_sample(const void* arena, Prec prec, const SampleJob* jobs, in
 constexpr int kTileBits = 12;
 constexpr int kTileHigh = kTileBits - 6;
 struct TileOp {
-     uint64_t mat_off;  // first complex entry of the op's 2^k x 2^k matrix
+     uint64_t mat_off;  // first complex entry of the op's 2^k x 2^k matrix (tile_mat_slot layout)
      uint8_t k;
      uint8_t rpos[5];  // ascending tile ranks; matrix bit l <-> rpos[l]
      uint8_t pad[2];
 };
+// Where entry i = o * D + j (row o, column j, D = 2^k) of a tile op's matrix
+// sits in the matrix array: row major for ops of 1-2 qubits; for 3-5 qubits
+// in the order the matrix-core passes read their A operands, so that lane l
+// of a wave reads entry s * 64 + l (k >= 4) at its s-th operand load, with
+// no per-load address arithmetic and consecutive lanes on consecutive LDS
+// banks:
+//   k = 4, 5: (o, j) at ((o / 16) * D / 4 + j / 4) * 64 + (j % 4) * 16 + o % 16
+//   k = 3:    (o, j) at (j / 4) * 32 + (j % 4) * 8 + o
+// The tile kernels read every op matrix through the same map.
+inline uint32_t tile_mat_slot(int k, uint32_t i)
+{
+     const uint32_t D = 1u << k, o = i >> k, j = i & (D - 1);
+     if (k >= 4) return ((o >> 4) * (D >> 2) + (j >> 2)) * 64 + (j & 3) * 16 + (o & 15);
+     if (k == 3) return (j >> 2) * 32 + (j & 3) * 8 + o;
+     return i;
+}
+// Writes a tile op's row-major 2^k x 2^k matrix m into dst (2^(2k) entries
+// of the state's precision) in the tile_mat_slot layout.
+inline void put_tile_matrix(void* dst, const std::complex<double>* m, int k, Prec prec)
+{
+     const uint32_t n = 1u << (2 * k);
+     for (uint32_t i = 0; i < n; ++i) {
+          const uint32_t s = tile_mat_slot(k, i);
+          if (prec == Prec::C128) {
+               double* d = static_cast<double*>(dst) + 2 * s;
+               d[0] = m[i].real();
+               d[1] = m[i].imag();
+          }
+          else {
+               float* d = static_cast<float*>(dst) + 2 * s;
+               d[0] = static_cast<float>(m[i].real());
+               d[1] = static_cast<float>(m[i].imag());
+          }
+     }
+}
 struct TileHigh {
      uint8_t hpos[kTileHigh];  // ascending index bits >= 6
 };

@@ -110,17 +110,17 @@ __device__ __forceinline__ uint32_t amp_offset(int j, const uint8_t (&pos)[5])
      return o;
 }
 
-// LDS slot of element r of a tile program's tile and of its staged op
-// matrices: the low four index bits XOR-ed with bits 4..7 (a bijection of
-// every aligned 256-element block).  LDS has 64 banks of 4 bytes, serving
-// a 64-lane 8-byte access in two 32-lane halves and a 16-byte access in four
-// 16-lane quarters.  In the identity layout the matrix-core operand loads
-// put lanes 2^rank or a 16-element matrix row apart on one bank (up to 16
-// accesses per bank in complex128); with this layout, modelled over the
-// ops of the 4096-circuit batch, the tile accesses need 0.79 (complex64) /
-// 0.54 (complex128) and the staged-matrix loads 0.33 / 0.11 of the bank
-// cycles.  It is linear over XOR, so for a group base b and an amplitude
-// offset o with disjoint bits, tslot(b + o) = tslot(b) ^ tslot(o).
+// LDS slot of element r of a tile program's tile: the low four index bits
+// XOR-ed with bits 4..7 (a bijection of every aligned 256-element block).
+// LDS has 64 banks of 4 bytes, serving a 64-lane 8-byte access in two
+// 32-lane halves and a 16-byte access in four 16-lane quarters.  In the
+// identity layout the matrix-core operand loads put lanes 2^rank apart on
+// one bank (up to 16 accesses per bank in complex128); with this layout,
+// modelled over the ops of the 4096-circuit batch, the tile accesses need
+// 0.79 (complex64) / 0.54 (complex128) of the bank cycles.  (The staged op
+// matrices need no swizzle: their layout, mat_slot, puts consecutive lanes
+// on consecutive entries.)  It is linear over XOR, so for a group base b and
+// an amplitude offset o with disjoint bits, tslot(b + o) = tslot(b) ^ tslot(o).
 __device__ __forceinline__ uint32_t tslot(uint32_t r)
 {
      return r ^ ((r >> 4) & 15u);
@@ -148,6 +148,21 @@ __device__ __forceinline__ uint32_t opaque_s(uint32_t x)
      return x;
 }
 
+// Entry (row o, column j) of a tile op's 2^K x 2^K matrix, in global memory
+// and in its LDS copy alike: row major for K <= 2; for K >= 3 the order in
+// which the matrix-core passes read their A operands (the host writes it so,
+// tile_mat_slot), where lane (q, c) = (l / 16, l % 16) reads M[16 rb + c][4 t + q]
+// at entry (rb D/4 + t) 64 + l (K >= 4) and M[o][4 t + q] at t 32 + q 8 + o
+// (K = 3).
+template <int K>
+__device__ __forceinline__ uint32_t mat_slot(uint32_t o, uint32_t j)
+{
+     constexpr uint32_t D = 1u << K;
+     if constexpr (K >= 4) return ((o >> 4) * (D >> 2) + (j >> 2)) * 64 + (j & 3) * 16 + (o & 15);
+     else if constexpr (K == 3) return (j >> 2) * 32 + (j & 3) * 8 + o;
+     else return o * D + j;
+}
+
 // Group index v spread over the non-target positions: a zero bit inserted
 // at each of pos[0] < ... < pos[K-1].  Bitwise, so index parts with disjoint
 // bits spread to disjoint bits: the addresses below split into a per-lane
@@ -167,8 +182,9 @@ __device__ __forceinline__ uint32_t insert_targets(uint32_t v, const uint8_t (&p
 
 // The same pass on a 2^N-amplitude LDS tile with the matrix read from
 // global memory at a uniform address (scalar loads into SGPRs, shared by the
-// whole wave), two rows at a time: no LDS traffic for the matrix, and no
-// SGPR spills (a fully unrolled row loop would hold the whole matrix).  When
+// whole wave; entries at mat_slot), two rows at a time: no LDS traffic for
+// the matrix, and no SGPR spills (a fully unrolled row loop would hold the
+// whole matrix).  When
 // the groups outnumber the threads, each thread reads all of its groups
 // before its first write, so their LDS latencies overlap.
 template <int K, typename R, int NTH, int GN, int N = kTileBits>
@@ -199,13 +215,12 @@ __device__ __forceinline__ void gm_groups(VT<R>* psi, const VT<R>* __restrict__ 
      }
 #pragma unroll 2
      for (int i = 0; i < D; ++i) {
-          const V* const row = mat + i * D;
 #pragma unroll
           for (int g = 0; g < G; ++g) {
                R re = 0, im = 0;
 #pragma unroll
                for (int j = 0; j < D; ++j) {
-                    const V m = row[j];
+                    const V m = mat[mat_slot<K>(static_cast<uint32_t>(i), static_cast<uint32_t>(j))];
                     re = fma(m.x, v[g][j].x, re);
                     re = fma(-m.y, v[g][j].y, re);
                     im = fma(m.x, v[g][j].y, im);
@@ -273,8 +288,8 @@ struct Mfma16<float> {
 };
 
 
-// mat[moff ..] is the op's matrix: in LDS at tslot(moff + index) when SW
-// (staged), else in global memory at moff + index.
+// mat[moff ..] is the op's matrix (mat_slot layout): in LDS when SW (staged),
+// else in global memory.
 template <int K, typename R, int NWV, int NCB, bool SW, int N = kTileBits>
 __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict__ mat, const uint32_t moff,
                                             const uint8_t (&pos)[5], const int w, const uint32_t cb0)
@@ -299,8 +314,7 @@ __device__ __forceinline__ void mfma_blocks(VT<R>* psi, const VT<R>* __restrict_
           const bool im_row = M::row_im(c);
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-               const uint32_t e = moff + static_cast<uint32_t>(o * D + 4 * t + q);
-               const V m = mat[SW ? tslot(e) : e];
+               const V m = mat[moff + mat_slot<K>(static_cast<uint32_t>(o), static_cast<uint32_t>(4 * t + q))];
                a[2 * t] = im_row ? m.y : m.x;       // times x.re
                a[2 * t + 1] = im_row ? m.x : -m.y;  // times x.im
           }
@@ -404,13 +418,13 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
      const int lane = threadIdx.x & 63;
      const int q = lane >> 4, c = lane & 15;
      // A fragments of row block rb: lane (q, c) holds Re, Im and Re + Im of
-     // M[16 rb + c][4 t + q].
+     // M[16 rb + c][4 t + q], at entry (rb T + t) 64 + lane (mat_slot), so
+     // the loads differ only by constant offsets.
+     const V* const alane = mat + moff + lane;
      auto load_a = [&](int rb, R (&a)[3 * T]) {
-          const int o = rb * 16 + c;
 #pragma unroll
           for (int t = 0; t < T; ++t) {
-               const uint32_t e = moff + static_cast<uint32_t>(o * D + 4 * t + q);
-               const V m = mat[SW ? tslot(e) : e];
+               const V m = alane[(rb * T + t) * 64];
                a[3 * t] = m.x;               // A, times u + v
                a[3 * t + 1] = m.x + m.y;     // A + B, times v
                a[3 * t + 2] = m.y - m.x;     // B - A, times u
@@ -470,9 +484,14 @@ __device__ __forceinline__ void mfma_blocks_3m(VT<R>* psi, const VT<R>* __restri
                          block(ahold, rb);
                     }
                     else {
+                         // one row block at a time (K = 5): interleaving the
+                         // two measured 2.5% slower on the 5-qubit one-op
+                         // passes (the operand loads were then issued just
+                         // before their use)
                          R a[3 * T];
                          load_a(rb, a);
                          block(a, rb);
+                         __builtin_amdgcn_sched_barrier(0);
                     }
                }
           }
@@ -628,8 +647,9 @@ struct TileIo {
 };
 
 // LDS copy of the matrices of one tile program (matrix-core variants): the
-// ops' matrices back to back in op order, as far as kStage elements go; ops
-// beyond that read global memory.
+// ops' matrices back to back in op order, as far as kStage elements go, in
+// their global layout (mat_slot: the operand loads of consecutive lanes hit
+// consecutive banks); ops beyond that read global memory.
 template <typename R>
 constexpr int tile_stage_elems()
 {
@@ -679,7 +699,7 @@ struct MatStage {
 #pragma unroll
           for (int i = 0; i < PER; ++i) {
                const int e = static_cast<int>(threadIdx.x) + i * NT;
-               if (e < n) smat[tslot(static_cast<uint32_t>(e))] = r[i];
+               if (e < n) smat[e] = r[i];
           }
      }
 };

@@ -578,6 +578,7 @@ bool Engine::dense5_as_tile(const Mat& m, const std::vector<uint32_t>& pos, uint
      // matrix bit l' <-> the l'-th lowest position (the tile op wants ascending ranks)
      int order[5] = {0, 1, 2, 3, 4};
      std::sort(order, order + 5, [&](int a, int b) { return pos[a] < pos[b]; });
+     // (entries in the tile kernels' matrix layout, tile_mat_slot)
      Mat ms(32);
      auto old_index = [&](size_t i) {
           size_t o = 0;
@@ -586,7 +587,8 @@ bool Engine::dense5_as_tile(const Mat& m, const std::vector<uint32_t>& pos, uint
           return o;
      };
      for (size_t i = 0; i < 32; ++i)
-          for (size_t j = 0; j < 32; ++j) ms(i, j) = m(old_index(i), old_index(j));
+          for (size_t j = 0; j < 32; ++j)
+               ms.a[tile_mat_slot(5, static_cast<uint32_t>(i * 32 + j))] = m(old_index(i), old_index(j));
      uint64_t high = 0;
      for (auto p: pos)
           if (p >= 6) high |= uint64_t(1) << p;
@@ -1180,18 +1182,7 @@ int Engine::tile_items(std::vector<PosGate>& items, int op_width, size_t keep)
                km = std::max(km, static_cast<int>(to.k));
                const size_t old = mats.size();
                mats.resize(old + M.a.size() * ab);
-               for (size_t i = 0; i < M.a.size(); ++i) {
-                    if (prec_ == Prec::C128) {
-                         double* d = reinterpret_cast<double*>(mats.data() + old) + 2 * i;
-                         d[0] = M.a[i].real();
-                         d[1] = M.a[i].imag();
-                    }
-                    else {
-                         float* d = reinterpret_cast<float*>(mats.data() + old) + 2 * i;
-                         d[0] = static_cast<float>(M.a[i].real());
-                         d[1] = static_cast<float>(M.a[i].imag());
-                    }
-               }
+               put_tile_matrix(mats.data() + old, M.a.data(), to.k, prec_);
                entries += M.a.size();
                ops.push_back(to);
           }

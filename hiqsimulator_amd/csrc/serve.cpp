@@ -264,19 +264,10 @@ void Server::planner_loop()
                     for (const auto& op: t.ops) entries += op.m.size();
                     std::vector<char> mb(entries * ab);
                     size_t o = 0;
-                    for (const auto& op: t.ops)
-                         for (size_t i = 0; i < op.m.size(); ++i, ++o) {
-                              if (prec_ == Prec::C128) {
-                                   double* d = reinterpret_cast<double*>(mb.data()) + 2 * o;
-                                   d[0] = op.m[i].real();
-                                   d[1] = op.m[i].imag();
-                              }
-                              else {
-                                   float* d = reinterpret_cast<float*>(mb.data()) + 2 * o;
-                                   d[0] = static_cast<float>(op.m[i].real());
-                                   d[1] = static_cast<float>(op.m[i].imag());
-                              }
-                         }
+                    for (const auto& op: t.ops) {
+                         put_tile_matrix(mb.data() + o * ab, op.m.data(), op.k, prec_);
+                         o += op.m.size();
+                    }
                     std::lock_guard<std::mutex> g(ps->m);
                     TPass pass{};
                     std::memcpy(pass.hpos, t.hpos, kTileHigh);
