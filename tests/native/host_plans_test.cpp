@@ -8,7 +8,8 @@
 //  * Batch::prepare() (plan, fuse, lay out and bounds-check every launch)
 //    on random mixed workloads under several plans, padded 2-D grids
 //    included;
-//  * error paths: gates beyond the register, too wide, repeated qubits.
+//  * error paths: gates beyond the register, too wide, repeated qubits;
+//  * the tile ops' matrix layout (tile_mat_slot) is a bijection.
 // Every plan is checked: each gate exactly once, widths within bounds, and
 // every pair of gates that share a qubit kept in order.
 #include <algorithm>
@@ -211,6 +212,29 @@ bool throws(F&& f)
      return false;
 }
 
+// The tile ops' matrix layout: a bijection of the 4^k entries for every
+// width, row major below 3 qubits, and each 64-entry operand block of the
+// 4-5-qubit layout holding one column quartet of 16 consecutive rows.
+void matrix_layouts()
+{
+     for (int k = 1; k <= 5; ++k) {
+          const uint32_t n = 1u << (2 * k), D = 1u << k;
+          std::vector<int> seen(n, 0);
+          for (uint32_t i = 0; i < n; ++i) {
+               const uint32_t s = hq::tile_mat_slot(k, i);
+               CHECK(s < n);
+               ++seen[s];
+               if (k <= 2) CHECK(s == i);
+               if (k >= 4) {
+                    const uint32_t o = i / D, j = i % D;
+                    CHECK(s / 64 == (o / 16) * (D / 4) + j / 4);
+                    CHECK(s % 64 == (j % 4) * 16 + o % 16);
+               }
+          }
+          for (uint32_t s = 0; s < n; ++s) CHECK(seen[s] == 1);
+     }
+}
+
 void errors()
 {
      hq::Batch b(hq::Prec::C128, 0);
@@ -263,6 +287,7 @@ int main()
      tile_plans(rng);
      batches(rng);
      errors();
+     matrix_layouts();
      std::printf("host plans ok\n");
      return 0;
 }
