@@ -564,16 +564,17 @@ void Engine::dense_or_diag(const Mat& m, const std::vector<uint32_t>& pos, uint6
      }
 }
 
-// A complex128 5-qubit pass without controls, on 12 or more qubits, runs as
-// a tile pass with one op: the tile holds bits 0..5 plus the targets at or
-// above bit 6 (filled up with the lowest other bits), and the op runs on the
-// matrix cores as three real products.  The dense forms keep the 32-amplitude
-// group in registers on the vector units; on a 30-qubit state the tile pass
-// takes 6.5 ms against 7.8 ms.  Timed as a 5-qubit dense pass.
+// A 5-qubit pass without controls, on 12 or more qubits, runs as a tile pass
+// with one op: the tile holds bits 0..5 plus the targets at or above bit 6
+// (filled up with the lowest other bits), and the op runs on the matrix cores
+// as three real products.  The dense forms keep the 32-amplitude group in
+// registers on the vector units; over random placements on a 30-qubit state
+// the tile pass takes 6.8 against 8.3 ms (complex128) and 3.2 against 4.4 ms
+// (complex64).  Timed as a 5-qubit dense pass.
 bool Engine::dense5_as_tile(const Mat& m, const std::vector<uint32_t>& pos, uint64_t ctrlmask)
 {
      const int n = n_qubits();
-     if (prec_ != Prec::C128 || pos.size() != 5 || ctrlmask != 0 || n < kTileBits || !get_dense5_tile())
+     if (pos.size() != 5 || ctrlmask != 0 || n < kTileBits || !get_dense5_tile())
           return false;
      // matrix bit l' <-> the l'-th lowest position (the tile op wants ascending ranks)
      int order[5] = {0, 1, 2, 3, 4};

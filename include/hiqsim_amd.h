@@ -283,7 +283,7 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *   "tile_mfma":          op engine of tile programs: 1 (default) ops of 3+
  *                         qubits on the matrix cores; 0 every op on the
  *                         vector units;
- *   "dense5_tile":        complex128 5-qubit passes without controls on 12+
+ *   "dense5_tile":        5-qubit passes without controls on 12+
  *                         qubits: 1 (default) as a one-op tile pass on the
  *                         matrix cores; 0 the dense pass forms.
  *   "tile_high_target_wide": the same threshold as "tile_high_target" for
