@@ -566,15 +566,17 @@ __global__ void __launch_bounds__(NT)
 // outside {0..5} u hpos are fixed: 64 chunks of 64 contiguous amplitudes,
 // chunk c at base + pdep(c, hpos).  One workgroup loads its tile into LDS,
 // runs the program's ops there (a workgroup barrier before each) and writes
-// the tile back; with two or three workgroups per CU, one workgroup's ops
-// overlap the others' HBM traffic (measured on a 30-qubit state: up to two
-// 4-qubit or four 3-qubit ops per tile cost no time over the bare copy).
+// the tile back; with two (complex128) or five (complex64) workgroups per
+// CU, one workgroup's ops overlap the others' HBM traffic (measured on a
+// 30-qubit state: up to two 4-qubit or four 3-qubit ops per tile cost no
+// time over the bare copy).
 //
 // Ops of at least MK qubits run on the matrix cores (wg_pass_mfma), the
 // others on the vector units with the matrix read by uniform scalar loads
-// (MK = 6: all on the vector units).  The matrix-core ops read their
-// matrices from an LDS copy (smat) loaded together with the tile, so the
-// op phase issues no global loads.
+// (MK = 6: all on the vector units).  In complex128 the matrix-core ops read
+// their matrices from an LDS copy (smat) loaded together with the tile, so
+// the op phase issues no global loads; complex64 reads them from global
+// memory and keeps the LDS for a fifth workgroup.
 // A descriptor read as 32-bit words at a wave-uniform address: the words
 // come through scalar loads (a byte field read on its own would be a vector
 // load, whose wait would also wait for the tile loads in flight).
@@ -653,14 +655,29 @@ struct TileIo {
 template <typename R>
 constexpr int tile_stage_elems()
 {
-     return 1024;  // 16 KiB complex128, 8 KiB complex64
+     // 16 KiB complex128.  Complex64 stages nothing: without the 8 KiB its
+     // 32 KiB tile lets five workgroups share a CU instead of four.
+     return sizeof(R) == 8 ? 1024 : 0;
+}
+
+// The LDS array for the staged matrices (none when N is 0).
+template <typename V, int N>
+__device__ __forceinline__ V* smat_buf()
+{
+     if constexpr (N > 0) {
+          __shared__ V buf[N];
+          return buf;
+     }
+     else {
+          return nullptr;
+     }
 }
 
 template <typename R, int NT>
 struct MatStage {
      using V = VT<R>;
      static constexpr int ME = tile_stage_elems<R>();
-     static constexpr int PER = ME / NT;
+     static constexpr int PER = ME / NT > 0 ? ME / NT : 1;
 
      // Loads the matrices of ops[0..nops) that fit into r; returns the
      // number of staged elements.  Addresses first (scalar walk over the
@@ -713,10 +730,11 @@ constexpr int tile_mfma_min_k(int mode) { return mode == kTileVector ? 6 : 3; }
 
 // Waves per SIMD the tile kernels are built for.  512-thread workgroups:
 // two per CU for complex128 (64 KiB tile + 16 KiB matrices; at most 128
-// VGPRs), four for complex64 (40 KiB; at most 64 VGPRs); programs with
-// 5-qubit ops get more registers (complex128 unconstrained, complex64 two
-// workgroups per CU): their 5-qubit passes need them.  256-thread
-// workgroups: the same workgroups per CU, so half the waves.
+// VGPRs), four for complex64 (at most 64 VGPRs; its 32 KiB of LDS would
+// allow five); programs with 5-qubit ops get more registers (complex128
+// unconstrained, complex64 two workgroups per CU): their 5-qubit passes
+// need them.  256-thread workgroups: half the waves per workgroup (the
+// complex64 ones stay within the registers of five per CU).
 template <typename R, int KMAX, int NT>
 constexpr int tile_waves_per_eu()
 {
@@ -789,7 +807,7 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
      V v[IO::PER];
      IO::load(psi, tix, hpos, v);
      int staged = 0;
-     constexpr bool kStage = MODE != kTileVector;
+     constexpr bool kStage = MODE != kTileVector && MS::ME > 0;
      V mr[kStage ? MS::PER : 1];
      if constexpr (kStage) staged = MS::load(mr, mats, ops, nops);
      IO::to_lds(tile, v);
@@ -809,7 +827,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_wa
                 const uint64_t nblocks)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[MODE != kTileVector ? tile_stage_elems<R>() : 1];
+     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R>() : 0>();
      const uint64_t b = block_linear();
      if (b >= nblocks) return;  // padding of the 2-D grid (whole block)
      tile_program<R, KMAX, NT, MODE>(psi, mats, ops, nops, h.hpos, swizzle_block(b, swz), tile, smat);
@@ -824,7 +842,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_wa
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     __shared__ VT<R> smat[MODE != kTileVector ? tile_stage_elems<R>() : 1];
+     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R>() : 0>();
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
      const BatchTileProg it = load_uniform(items + find_item(items, nitems, b));
