@@ -575,8 +575,9 @@ __global__ void __launch_bounds__(NT)
 // others on the vector units with the matrix read by uniform scalar loads
 // (MK = 6: all on the vector units).  In complex128 the matrix-core ops read
 // their matrices from an LDS copy (smat) loaded together with the tile, so
-// the op phase issues no global loads; complex64 reads them from global
-// memory and keeps the LDS for a fifth workgroup.
+// the op phase issues no global loads; complex64 programs of at most
+// 4-qubit ops read them from global memory and keep the LDS for a fifth
+// workgroup.
 // A descriptor read as 32-bit words at a wave-uniform address: the words
 // come through scalar loads (a byte field read on its own would be a vector
 // load, whose wait would also wait for the tile loads in flight).
@@ -652,12 +653,15 @@ struct TileIo {
 // ops' matrices back to back in op order, as far as kStage elements go, in
 // their global layout (mat_slot: the operand loads of consecutive lanes hit
 // consecutive banks); ops beyond that read global memory.
-template <typename R>
+template <typename R, int KMAX>
 constexpr int tile_stage_elems()
 {
-     // 16 KiB complex128.  Complex64 stages nothing: without the 8 KiB its
-     // 32 KiB tile lets five workgroups share a CU instead of four.
-     return sizeof(R) == 8 ? 1024 : 0;
+     // 16 KiB complex128.  Complex64 programs of at most 4-qubit ops stage
+     // nothing: without the 8 KiB their 32 KiB tile lets five workgroups
+     // share a CU instead of four.  With 5-qubit ops the registers allow
+     // four at most, and the 8 KiB copy is kept (one-op 5-qubit passes on
+     // 30 qubits: 3.12-3.21 ms staged, 3.44-3.46 ms not).
+     return sizeof(R) == 8 || KMAX >= 5 ? 1024 : 0;
 }
 
 // The LDS array for the staged matrices (none when N is 0).
@@ -673,10 +677,10 @@ __device__ __forceinline__ V* smat_buf()
      }
 }
 
-template <typename R, int NT>
+template <typename R, int NT, int KMAX>
 struct MatStage {
      using V = VT<R>;
-     static constexpr int ME = tile_stage_elems<R>();
+     static constexpr int ME = tile_stage_elems<R, KMAX>();
      static constexpr int PER = ME / NT > 0 ? ME / NT : 1;
 
      // Loads the matrices of ops[0..nops) that fit into r; returns the
@@ -803,7 +807,7 @@ __device__ __forceinline__ void tile_program(VT<R>* __restrict__ psi, const VT<R
 {
      using V = VT<R>;
      using IO = TileIo<R, NT>;
-     using MS = MatStage<R, NT>;
+     using MS = MatStage<R, NT, KMAX>;
      V v[IO::PER];
      IO::load(psi, tix, hpos, v);
      int staged = 0;
@@ -827,7 +831,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_wa
                 const uint64_t nblocks)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R>() : 0>();
+     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R, KMAX>() : 0>();
      const uint64_t b = block_linear();
      if (b >= nblocks) return;  // padding of the 2-D grid (whole block)
      tile_program<R, KMAX, NT, MODE>(psi, mats, ops, nops, h.hpos, swizzle_block(b, swz), tile, smat);
@@ -842,7 +846,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(tile_wa
                   const TileOp* __restrict__ ops, const uint64_t total_blocks, const int swz)
 {
      __shared__ VT<R> tile[1 << kTileBits];
-     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R>() : 0>();
+     VT<R>* const smat = smat_buf<VT<R>, MODE != kTileVector ? tile_stage_elems<R, KMAX>() : 0>();
      const uint64_t b = block_linear();
      if (b >= total_blocks) return;  // padding of the 2-D grid (whole block)
      const BatchTileProg it = load_uniform(items + find_item(items, nitems, b));
