@@ -18,7 +18,8 @@ cpu_baseline = the reference's own AVX2 kernels (oracle/_ref, OpenMP with
                median of 5 passes on the 2^n-amplitude complex128 state, and
                the first 36 of the headline circuit's fused passes (about
                11 s), the whole circuit's time projected from them.
-Also reported: the step at every fusion width (circuit_k_sweep), the drop-in
+Also reported: single k-qubit passes at random placements in both precisions
+(k_sweep_gbs, k_sweep_gbs_c64), the step at every fusion width (circuit_k_sweep), the drop-in
 flow (run() per cluster with deferred tile passes), tile programs, the
 QFT-like precision round trip, the 4096-circuit batch and the serving
 streams (p50/p99 with 95% bootstrap intervals, served-state fidelity).
@@ -145,13 +146,14 @@ def inverse_step(step, start_order):
     return out
 
 
-def kernel_sweep(hq, n, reps=24):
+def kernel_sweep(hq, n, reps=24, precision="c128"):
     """Per-pass GB/s of the dense kernel for k = 1..5, averaged over `reps`
     random placements (a pass's time depends on where its targets lie)."""
     from hiqsimulator_amd.circuits import haar_unitary
     rng = np.random.default_rng(5)
-    sim = hq.SimulatorMPI(1, n, 5)
+    sim = hq.SimulatorMPI(1, n, 5, precision)
     sim.allocate_qureg(list(range(n)), 0)
+    amp_bytes = 16 if precision == "c128" else 8
     out = {}
     for k in range(1, 6):
         mats = [haar_unitary(1 << k, rng) for _ in range(reps)]
@@ -163,7 +165,7 @@ def kernel_sweep(hq, n, reps=24):
             sim.apply_matrix(m, p)
         cnt, ms = sim.kernel_time(k - 1)
         sim.set_timing(False)
-        out[str(k)] = round(32.0 * 2 ** n / (ms / cnt * 1e-3) / 1e9, 1)
+        out[str(k)] = round(2.0 * amp_bytes * 2 ** n / (ms / cnt * 1e-3) / 1e9, 1)
     del sim
     return out
 
@@ -747,6 +749,9 @@ def main():
     sweep = None if args.no_sweep else kernel_sweep(hq, n)
     if sweep:
         log("per-pass GB/s by k: %s" % sweep)
+    sweep64 = None if args.no_sweep else kernel_sweep(hq, n, precision="c64")
+    if sweep64:
+        log("per-pass GB/s by k, complex64: %s" % sweep64)
     drop_in = None if args.no_drop_in else bench_drop_in(hq, n, args.layers, k, args.seed, args.steps, log)
     circuit_sweep = None
     if not args.no_circuit_sweep:
@@ -788,6 +793,7 @@ def main():
                   "gates_per_s": round(len(gates) / (ms_step * 1e-3), 1),
                   "permutations_per_s": round(n_perm / (ms_step * 1e-3), 2)},
         "k_sweep_gbs": sweep,
+        "k_sweep_gbs_c64": sweep64,
         "circuit_k_sweep": circuit_sweep,
         "tile_planner": tile,
         "drop_in": drop_in,
