@@ -1133,7 +1133,7 @@ hipError_t launch_batch_tprog(void* arena, Prec prec, const void* mats, const Ba
      const dim3 grid = grid_2d(total_blocks);
      const int swz = get_block_swizzle();
      const int mode = get_tile_mfma();
-     const bool narrow = tile_threads_for(prec) == 256;
+     const bool narrow = tile_threads_for(prec, kmax) == 256;
 #define HQ_BTP(RR, KM, NT, MODE)                                                                   \
      hipLaunchKernelGGL((k_batch_tprog<RR, KM, NT, MODE>), grid, dim3(NT), 0, s,                   \
                         static_cast<VT<RR>*>(arena), static_cast<const VT<RR>*>(mats), items,     \
@@ -1168,7 +1168,7 @@ hipError_t launch_tile_pass(void* psi, Prec prec, int n_qubits, const TileHigh& 
      const int swz = (bits > 0 && bits <= 16 && blocks % (1ull << bits) == 0) ? bits : 0;
      const dim3 grid = grid_2d(blocks);
      const int mode = get_tile_mfma();
-     const bool narrow = tile_threads_for(prec) == 256;
+     const bool narrow = tile_threads_for(prec, kmax) == 256;
 #define HQ_TP(RR, KM, NT, MODE)                                                                    \
      hipLaunchKernelGGL((k_tile_pass<RR, KM, NT, MODE>), grid, dim3(NT), 0, s,                     \
                         static_cast<VT<RR>*>(psi), static_cast<const VT<RR>*>(mats), ops, nops, h, \

@@ -621,7 +621,9 @@ std::atomic<int> g_permute_block{256};
 // ms with 256 threads against 376 with 512 (the per-op lane setup is shared
 // by four column blocks per wave instead of two); complex128 693 against
 // 680 ms, and the 30-qubit tile pass 8.07 against 7.85 ms (two workgroups
-// per CU leave too few waves at 256).
+// per CU leave too few waves at 256).  Complex128 programs with 5-qubit ops
+// take 256: one-op 5-qubit passes 6.67 against 6.85 ms on 30 qubits (each
+// wave then holds two 32-row column blocks that share its operand loads).
 std::atomic<int> g_tile_threads{0};
 
 // Swizzle width for a grid of `blocks` blocks of a k-qubit pass: the
@@ -829,10 +831,10 @@ void set_permute_block(int bs) { g_permute_block.store(bs == 512 ? 512 : 256, kR
 int get_permute_block() { return g_permute_block.load(kRelaxed); }
 void set_tile_threads(int nt) { g_tile_threads.store(nt == 256 || nt == 512 ? nt : 0, kRelaxed); }
 int get_tile_threads() { return g_tile_threads.load(kRelaxed); }
-int tile_threads_for(Prec p)
+int tile_threads_for(Prec p, int kmax)
 {
      const int nt = get_tile_threads();
-     return nt ? nt : (p == Prec::C64 ? 256 : 512);
+     return nt ? nt : (p == Prec::C64 || kmax >= 5 ? 256 : 512);
 }
 
 hipError_t launch_dense(void* psi, Prec prec, int n_qubits, const void* mat, const DenseDesc& d,

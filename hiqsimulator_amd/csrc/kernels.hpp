@@ -75,10 +75,11 @@ int get_permute_swizzle();
 void set_permute_block(int bs);
 int get_permute_block();
 // Threads per workgroup of the tile-program kernels with matrix-core ops:
-// 256, 512 or 0 (the default: 256 for complex64, 512 for complex128).
+// 256, 512 or 0 (the default: 256 for complex64 and for complex128 programs
+// with 5-qubit ops, 512 for the other complex128 programs).
 void set_tile_threads(int nt);
 int get_tile_threads();
-int tile_threads_for(Prec p);
+int tile_threads_for(Prec p, int kmax);
 
 // psi[i] *= diag[d(i)] for every i whose control bits are 1; d(i) gathers the
 // target bits.  diag holds 2^k complex128 values (k <= 5).

@@ -289,7 +289,14 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *   "tile_high_target_wide": the same threshold as "tile_high_target" for
  *                         4-qubit passes (default 13);
  *   "direct_block":       threads per block of the direct form of 4-qubit
- *                         passes: 256, 512 or 1024 (default).
+ *                         passes: 256, 512 or 1024 (default);
+ *   "tile_threads":       threads per workgroup of the tile-program kernels
+ *                         on the matrix cores: 0 (default: 256 for complex64
+ *                         and for complex128 programs with 5-qubit ops, 512
+ *                         for the other complex128 programs), 256 or 512;
+ *   "permute_swizzle":    low block-index bits reversed in the tiled qubit
+ *                         permutation (default 0, at most 16);
+ *   "permute_block":      its threads per workgroup: 256 (default) or 512.
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);
 int hiqsim_get_tuning(const char* name, int64_t* value);
