@@ -81,3 +81,23 @@ def test_tuning_parameters_round_trip():
             m.set_tuning(name, old)
     with pytest.raises(RuntimeError):
         m.get_tuning("no_such_knob")
+
+
+def test_enumerated_tuning_parameters():
+    """Knobs with a set of values: each value reads back, a value outside
+    the set raises and leaves the setting as it was."""
+    import pytest
+    from hiqsimulator_amd import _cppsim_amd as m
+    sets = {"tile_threads": (0, 256, 512), "direct_block": (256, 512, 1024), "permute_block": (256, 512)}
+    for name, values in sets.items():
+        old = m.get_tuning(name)
+        try:
+            for v in values:
+                m.set_tuning(name, v)
+                assert m.get_tuning(name) == v
+            for bad in (-1, 300, 2048):
+                with pytest.raises(RuntimeError):
+                    m.set_tuning(name, bad)
+                assert m.get_tuning(name) == values[-1]
+        finally:
+            m.set_tuning(name, old)

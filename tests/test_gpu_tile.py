@@ -68,6 +68,35 @@ def test_tile_ops_on_matrix_cores_and_vector_units(engine_mod, oracle_mod, mfma,
         mod.set_tuning("tile_mfma", old)
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+def test_tile_workgroup_sizes(engine_mod, oracle_mod, threads):
+    """Both workgroup sizes of the matrix-core tile programs (the default
+    picks 256 threads for complex64 and for complex128 programs with 5-qubit
+    ops, 512 otherwise): op widths 4 and 5, both precisions, against the
+    oracle."""
+    import hiqsimulator_amd
+    mod = hiqsimulator_amd._cppsim_amd
+    old = mod.get_tuning("tile_threads")
+    n = 15
+    try:
+        mod.set_tuning("tile_threads", threads)
+        for op_width in (4, 5):
+            g = circuit(n, 10, 400 + op_width)
+            ref = oracle_state(oracle_mod, n, g).get_state()
+            for prec in ("c128", "c64"):
+                print("tile threads=%d op_width=%d %s" % (threads, op_width, prec), flush=True)
+                sim = engine_mod.SimulatorMPI(1, n, 5, prec)
+                sim.allocate_qureg(list(range(n)), 0)
+                sim.apply_tiled(g, op_width)
+                got = sim.get_state()
+                if prec == "c128":
+                    assert np.abs(got - ref).max() <= 1e-12
+                else:
+                    assert fidelity(ref, got) >= 1 - 1e-6
+    finally:
+        mod.set_tuning("tile_threads", old)
+
+
 def test_tiled_after_permutation_and_pending_gates(engine_mod, oracle_mod):
     """apply_tiled after permute_qubits with gates still pending: the pending
     fusion (at most 5 qubits, as the reference's scheduler keeps it) runs
