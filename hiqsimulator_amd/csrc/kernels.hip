@@ -613,8 +613,12 @@ std::atomic<int> g_dense5_tile{1};
 std::atomic<int> g_high_target_wide{13};
 std::atomic<int> g_direct_block{1024};
 // Tiled qubit permutation: low block-index bits reversed, and threads per
-// workgroup (256 or 512).
-std::atomic<int> g_permute_swizzle{0};
+// workgroup (256 or 512).  Measured on 16 random 30-qubit permutations, the
+// same sequence for each setting (3 rounds): complex128 6.27 ms without the
+// swizzle, 6.08 ms with 8 bits (5477 against 5650 GB/s), 6.37 ms with 8
+// bits and 512 threads; complex64 3.55 against 3.53 ms.  (A 13-bit tile in
+// 1024-thread workgroups, 2 KiB contiguous reads, measured 6.11 ms.)
+std::atomic<int> g_permute_swizzle{8};
 std::atomic<int> g_permute_block{256};
 // Threads per workgroup of the tile-program kernels on the matrix cores: 0
 // picks per precision.  Measured on the 4096-circuit batch: complex64 359

@@ -295,7 +295,7 @@ int hiqsim_plan_tile_passes(const int64_t* gate_pos, const size_t* gate_off, con
  *                         and for complex128 programs with 5-qubit ops, 512
  *                         for the other complex128 programs), 256 or 512;
  *   "permute_swizzle":    low block-index bits reversed in the tiled qubit
- *                         permutation (default 0, at most 16);
+ *                         permutation (default 8, at most 16);
  *   "permute_block":      its threads per workgroup: 256 (default) or 512.
  * Batches and request loops read them when they build or launch. */
 int hiqsim_set_tuning(const char* name, int64_t value);

@@ -93,6 +93,23 @@ def test_dense_and_diag_c64(engine_mod, oracle_mod, n):
             assert err <= 2e-6, (n, k, diag, err)
 
 
+@pytest.mark.parametrize("knobs", [(8, 256), (0, 256), (6, 512)])
+def test_permutation_knobs_bit_exact(engine_mod, knobs):
+    """The tiled permutation with and without its block swizzle and in both
+    workgroup sizes: the result is the closed form out[sigma(i)] = in[i]."""
+    import hiqsimulator_amd
+    mod = hiqsimulator_amd._cppsim_amd
+    old = (mod.get_tuning("permute_swizzle"), mod.get_tuning("permute_block"))
+    try:
+        mod.set_tuning("permute_swizzle", knobs[0])
+        mod.set_tuning("permute_block", knobs[1])
+        for precision in ("c128", "c64"):
+            test_permutation_bit_exact(engine_mod, 20, precision)
+    finally:
+        mod.set_tuning("permute_swizzle", old[0])
+        mod.set_tuning("permute_block", old[1])
+
+
 @pytest.mark.parametrize("precision", ["c128", "c64"])
 @pytest.mark.parametrize("n", [1, 3, 8, 15, 21])
 def test_permutation_bit_exact(engine_mod, n, precision):
