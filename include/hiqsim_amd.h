@@ -183,7 +183,9 @@ int hiqsim_sample(hiqsim_t* h, uint32_t shots, uint64_t seed, uint64_t* out);
  * rejected with an error).  Intermediates (2^15 block probabilities and two
  * descriptors) live in one static device array of 256.5 KiB per GPU, held
  * until the process ends and shared by every call on that GPU: calls that
- * overlap on one GPU (several streams or host threads) race. */
+ * overlap on one GPU (several streams or host threads) race.  The call may be
+ * captured into a HIP graph once it has run eagerly on that device (its first
+ * call on a device looks up the array's address). */
 int hiqsim_sample_noalloc(const void* d_state, int n_qubits, int precision, uint32_t shots,
                           uint64_t seed, uint64_t* d_out, void* stream);
 

@@ -114,6 +114,83 @@ def test_noalloc_sampler_matches_engine_sampler(engine_mod, prec):
         assert int(got.max()) < (1 << n)
 
 
+def _hip_graph_api(hip):
+    vp, pvp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)
+    hip.hipStreamCreate.argtypes = [pvp]
+    hip.hipStreamDestroy.argtypes = [vp]
+    hip.hipStreamSynchronize.argtypes = [vp]
+    hip.hipStreamBeginCapture.argtypes = [vp, ctypes.c_int]
+    hip.hipStreamEndCapture.argtypes = [vp, pvp]
+    hip.hipGraphInstantiate.argtypes = [pvp, vp, vp, ctypes.c_char_p, ctypes.c_size_t]
+    hip.hipGraphLaunch.argtypes = [vp, vp]
+    hip.hipGraphExecDestroy.argtypes = [vp]
+    hip.hipGraphDestroy.argtypes = [vp]
+
+
+def test_noalloc_sampler_under_graph_capture(engine_mod):
+    """The entry point captured into a HIP graph on the caller's stream (the
+    use it is for: no allocation may happen while a stream is captured).
+    Three replays, with an eager call for another state size and seed
+    between them (it leaves other contents in the shared scratch array),
+    each give the outcomes of hiqsim_sample."""
+    lib, hip = _lib(engine_mod), _hip()
+    _hip_graph_api(hip)
+    n, shots, seed = 16, 2048, 5
+    h = _engine(lib, n, C128, 303)
+    h2 = _engine(lib, 12, C128, 404)
+    stream, graph, gexec = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        ref = (ctypes.c_uint64 * shots)()
+        _ok(lib, lib.hiqsim_sample(h, shots, seed, ref))
+        ref = np.ctypeslib.as_array(ref).copy()
+        ref2 = (ctypes.c_uint64 * 300)()
+        _ok(lib, lib.hiqsim_sample(h2, 300, 9, ref2))
+        ref2 = np.ctypeslib.as_array(ref2).copy()
+        ptr, ptr2 = ctypes.c_void_p(), ctypes.c_void_p()
+        amps, p, dev = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_int()
+        _ok(lib, lib.hiqsim_pin_device_state(h, ctypes.byref(ptr), ctypes.byref(amps), ctypes.byref(p),
+                                             ctypes.byref(dev)))
+        _ok(lib, lib.hiqsim_pin_device_state(h2, ctypes.byref(ptr2), ctypes.byref(amps), ctypes.byref(p),
+                                             ctypes.byref(dev)))
+        assert hip.hipStreamCreate(ctypes.byref(stream)) == 0
+        with DeviceArray(hip, shots * 8) as out, DeviceArray(hip, 300 * 8) as out2:
+            # an eager call first: the entry point looks up its scratch
+            # array's address on its first call on a device
+            _ok(lib, lib.hiqsim_sample_noalloc(ptr, n, C128, shots, seed, out.ptr, stream))
+            assert hip.hipStreamSynchronize(stream) == 0
+            print("sample_noalloc: capture n=%d shots=%d" % (n, shots), flush=True)
+            assert hip.hipStreamBeginCapture(stream, 0) == 0  # hipStreamCaptureModeGlobal
+            rc = lib.hiqsim_sample_noalloc(ptr, n, C128, shots, seed, out.ptr, stream)
+            assert hip.hipStreamEndCapture(stream, ctypes.byref(graph)) == 0
+            _ok(lib, rc)
+            assert hip.hipGraphInstantiate(ctypes.byref(gexec), graph, None, None, 0) == 0
+            got, got2 = np.zeros(shots, dtype=np.uint64), np.zeros(300, dtype=np.uint64)
+            for replay in range(3):
+                print("sample_noalloc: replay %d" % replay, flush=True)
+                assert hip.hipMemset(out.ptr, 0xFF, shots * 8) == 0
+                assert hip.hipDeviceSynchronize() == 0
+                assert hip.hipGraphLaunch(gexec, stream) == 0
+                assert hip.hipStreamSynchronize(stream) == 0
+                assert hip.hipMemcpy(got.ctypes.data, out.ptr, shots * 8, DEVICE_TO_HOST) == 0
+                assert np.array_equal(got, ref), (replay, int(np.sum(got != ref)))
+                # eager work between the replays
+                _ok(lib, lib.hiqsim_sample_noalloc(ptr2, 12, C128, 300, 9, out2.ptr, None))
+                assert hip.hipDeviceSynchronize() == 0
+                assert hip.hipMemcpy(got2.ctypes.data, out2.ptr, 300 * 8, DEVICE_TO_HOST) == 0
+                assert np.array_equal(got2, ref2), replay
+        _ok(lib, lib.hiqsim_unpin_device_state(h))
+        _ok(lib, lib.hiqsim_unpin_device_state(h2))
+    finally:
+        if gexec.value:
+            hip.hipGraphExecDestroy(gexec)
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        if stream.value:
+            hip.hipStreamDestroy(stream)
+        lib.hiqsim_destroy(h)
+        lib.hiqsim_destroy(h2)
+
+
 def test_noalloc_sampler_rejects_out_of_range(engine_mod):
     lib, hip = _lib(engine_mod), _hip()
     with DeviceArray(hip, 32) as state, DeviceArray(hip, 4 * 8) as out:
